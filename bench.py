@@ -1,0 +1,191 @@
+"""Benchmark: candidate evaluations per second of the full six-stage chain on one MI355X.
+
+One step = one ce_evaluate() call over the whole candidate block of the workload (inputs already
+resident in HBM), i.e. simulation, linearised eigensolve, modal fit, margin sweep and scoring of
+every candidate.  The default workload is the 4-state two-mass drive (65,536 candidates, T=5000,
+F=400); --config c3 selects the 8-state, 262,144-candidate case.
+
+Printed: one JSON line with the throughput of the headline mode, the FP64 mode beside it, the
+roofline of the dominant kernel (HIP events on the launch stream) and the CPU oracle timed on the
+host cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_flops(problem, info64, d):
+    """Algorithmic flops per stage for the candidates that reached each stage (formulas of the
+    measurement plan: FMA = 2, add/mul/div = 1, each non-SKIP instruction = 1)."""
+    import numpy as np
+    c = problem.config
+    ns, nd, nw = int(c["statespace_dim"]), int(c["discrete_dim"]), int(c["nonlin_dim"])
+    T, F = int(c["time"]), int(c["freq"])
+    nt = ns + nd
+    ninstr = int(c["dc_instructions"]) + int(c["u_instructions"]) + sum(c["nl_instructions"] or [0])
+    sc0 = info64[:, d.info_scoring]
+    unstable = (info64[:, d.info_scoring + 1] >= float(c["unstable_err"])) & (info64[:, d.info_allmargin + 3] == 0)
+    uniq = info64[:, d.info_modal + 3 * nt]
+    n = info64.shape[0]
+    per_step = 4 * ns + 5 + ns * (2 * ns + 4) + nd * (2 * nd + 2 * nw + 2) + ninstr
+    sim = float(per_step) * T * int((~unstable).sum())
+    eig = float(10 * nt ** 3 + 4 * ns * ns + 2 * ns * nd + 2 * ns) * n
+    fitted = uniq > 0
+    u = uniq[fitted]
+    nc = 2 * u
+    tfit = d.fit_count
+    fit = float(np.sum(tfit * (nc * (nc + 1) + 2 * nc + 4 * u + 1) + 2 * T))
+    solve = float(np.sum((2 * nt) ** 3 / 3 + 4 * (2 * nt) ** 2)) * int(fitted.sum()) / max(1, int(fitted.sum()))
+    swept = info64[:, d.info_allmargin + 3] == 1
+    sweep = float(F * (8.0 / 3.0 * ns ** 3 + 8 * ns * ns + 8 * ns + 30) + 10 * (2 * ns) ** 3) * int(swept.sum())
+    return dict(eig=eig, sim=sim, fit=fit + solve, sweep=sweep)
+
+
+def cpu_baseline(problem, target_s=12.0, max_n=None):
+    """Oracle (FP64 restatement, OpenMP over candidates) on a bounded sample of the workload."""
+    import numpy as np
+    from oracle import oracle
+    threads = int(os.environ.get("CE_CPU_THREADS", "0")) or oracle.lib().ora_max_threads()
+    n_total = problem.gains.shape[0]
+    n = min(256, n_total)
+    t = 0.0
+    while True:
+        sub = problem.subset(np.arange(n))
+        t0 = time.perf_counter()
+        oracle.evaluate(sub, threads=threads)
+        t = time.perf_counter() - t0
+        if t >= 0.5 * target_s / 4 or n >= n_total or (max_n and n >= max_n):
+            break
+        n = min(n_total, n * 4)
+    # final timed run sized to about target_s
+    n_final = int(min(n_total, max(n, n * target_s / max(t, 1e-3))))
+    if max_n:
+        n_final = min(n_final, max_n)
+    sub = problem.subset(np.arange(n_final))
+    t0 = time.perf_counter()
+    oracle.evaluate(sub, threads=threads)
+    t = time.perf_counter() - t0
+    return dict(value=n_final / t, unit="candidate-evals/s", cores=threads, kind="port",
+                sample="first %d of %d candidates, FP64 oracle, %.1f s" % (n_final, n_total, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--n", type=int, default=0, help="candidates (default: the configuration's count)")
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the candidate chain has no cross-GPU step in this build "
+              "(multi-GPU sharding is by candidate range, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from controllerengine_amd import PRECISION_FP64, PRECISION_MIXED, workloads
+    from controllerengine_amd.engine import Engine
+
+    gen = workloads.CONFIGS[args.config]
+    problem = gen(n=args.n) if args.n else gen()
+    n = problem.gains.shape[0]
+    d = problem.derived()
+    eng = Engine(0)
+    cfg = problem.cfg_struct()
+    plant = problem.plant_struct(0)
+    g = eng.to_device(problem.gains)
+    i_in = eng.to_device(problem.info)
+    i_out = torch.empty_like(i_in)
+    i64 = torch.empty(i_in.shape, dtype=torch.float64, device=i_in.device)
+
+    modes = []
+    have_mixed = True
+    if args.precision in ("auto", "mixed"):
+        try:
+            eng.evaluate_device(cfg, plant, PRECISION_MIXED, g, i_in, i_out, None)
+            torch.cuda.synchronize()
+        except RuntimeError:
+            have_mixed = False
+    if args.precision == "fp64" or not have_mixed:
+        modes = [("fp64", PRECISION_FP64)]
+    elif args.precision == "mixed":
+        modes = [("mixed", PRECISION_MIXED)]
+    else:
+        modes = [("mixed", PRECISION_MIXED), ("fp64", PRECISION_FP64)]
+
+    results = {}
+    for name, prec in modes:
+        for _ in range(args.warmup):
+            eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
+        torch.cuda.synchronize()
+        stage_acc = None
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
+            st = eng.stage_times()  # synchronises on the step's last event
+            stage_acc = st if stage_acc is None else [a + b for a, b in zip(stage_acc, st)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        info64 = i64.cpu().numpy()
+        stages = [s / args.steps for s in stage_acc]
+        results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, stages=stages, info64=info64)
+
+    head = modes[0][0]
+    r = results[head]
+    fl = stage_flops(problem, r["info64"], d)
+    names = ["eig", "sim", "fit", "sweep", "final"]
+    stage_ms = dict(zip(names, r["stages"]))
+    dom = max(["eig", "sim", "fit", "sweep"], key=lambda k: stage_ms[k])
+    achieved = fl[dom] / (stage_ms[dom] * 1e-3) / 1e12
+    peak = FP64_PEAK_TFLOPS
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (dom, args.config, head))
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": "candidate evals/s, mixed vs FP64 plain at matched margins; p50/p99 job latency",
+        "value": r["value"],
+        "unit": "candidate-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s, %d candidates, T=%d, F=%d, %s" % (
+            args.config, {"c1": "mass-spring-damper 2-state", "c2": "two-mass drive 4-state, 2 NL",
+                          "c3": "random stable 8-state, 4 NL"}[args.config], n, cfg.time, cfg.freq, head),
+                   "precision": head},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "flops_per_launch": fl[dom]},
+    }
+    if "fp64" in results and head != "fp64":
+        line["fp64_value"] = results["fp64"]["value"]
+        line["speedup_vs_fp64"] = r["value"] / results["fp64"]["value"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(problem, target_s=args.cpu_seconds)
+    eng.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

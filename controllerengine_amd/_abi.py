@@ -1,0 +1,123 @@
+"""ctypes mirror of include/ControllerEngine.h and the library loader.
+
+The product path is the HIP library libControllerEngine.so built in-tree by
+controllerengine_amd/build.py; loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libControllerEngine.so")
+
+MAX_SS = 8
+MAX_DS = 8
+MAX_NONLIN = 4
+MAX_TOTAL = 16
+MAX_NL_DIM = 9
+MAX_DC = 4
+MAX_U = 4
+MAX_NL = 8
+MAX_PARAMS = 34
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+class CeConfig(ctypes.Structure):
+    _fields_ = [
+        ("statespace_dim", ctypes.c_int32), ("discrete_dim", ctypes.c_int32), ("nonlin_dim", ctypes.c_int32),
+        ("n_params", ctypes.c_int32), ("input_size", ctypes.c_int32),
+        ("dc_instructions", ctypes.c_int32), ("u_instructions", ctypes.c_int32),
+        ("nl_instructions", ctypes.c_int32 * MAX_NONLIN),
+        ("n_cores", ctypes.c_int32), ("n_rounds", ctypes.c_int32), ("serial", ctypes.c_int32),
+        ("time", ctypes.c_int32), ("duration", ctypes.c_float), ("ref_step_time", ctypes.c_double),
+        ("ref_amplitude", ctypes.c_float),
+        ("fit_start", ctypes.c_float), ("fit_end", ctypes.c_float), ("steady_start", ctypes.c_float),
+        ("bandwidth", ctypes.c_float), ("damping", ctypes.c_float), ("overshoot", ctypes.c_float),
+        ("steady_state_error", ctypes.c_float), ("crossover_coeff", ctypes.c_float),
+        ("target_gm", ctypes.c_float), ("target_pm", ctypes.c_float), ("target_b", ctypes.c_float),
+        ("eig_fail_err", ctypes.c_float), ("unstable_err", ctypes.c_float),
+        ("freq", ctypes.c_int32), ("freq_min", ctypes.c_float), ("dump_diagnostics", ctypes.c_int32),
+    ]
+
+
+class CePlant(ctypes.Structure):
+    _fields_ = [
+        ("ss_a", ctypes.c_float * (MAX_SS * MAX_SS)), ("ss_a_i", ctypes.c_float * (MAX_SS * MAX_SS)),
+        ("ss_b", ctypes.c_float * MAX_SS), ("ss_c", ctypes.c_float * MAX_SS), ("ss_d", ctypes.c_float),
+        ("d_a", ctypes.c_float * (MAX_DS * MAX_DS)), ("d_a_i", ctypes.c_float * (MAX_DS * MAX_DS)),
+        ("d_bu", ctypes.c_float * MAX_DS), ("d_bw", ctypes.c_float * (MAX_DS * MAX_NONLIN)),
+        ("d_c", ctypes.c_float * MAX_DS), ("base_params", ctypes.c_float * MAX_PARAMS),
+        ("dc_info", ctypes.c_int32 * (MAX_DC * MAX_DS)), ("u_info", ctypes.c_int32 * (MAX_U * MAX_SS)),
+        ("nl_info", ctypes.c_int32 * (MAX_NONLIN * MAX_NL * MAX_NL_DIM)),
+    ]
+
+
+class CeDerived(ctypes.Structure):
+    _fields_ = [
+        ("n_threads", ctypes.c_int32), ("total_dim", ctypes.c_int32), ("cuda_mx_width", ctypes.c_int32),
+        ("info_mx_width", ctypes.c_int32), ("info_scoring", ctypes.c_int32), ("info_freqresp", ctypes.c_int32),
+        ("info_allmargin", ctypes.c_int32), ("info_modal", ctypes.c_int32), ("fit_start_id", ctypes.c_int32),
+        ("steady_start_id", ctypes.c_int32), ("fit_count", ctypes.c_int32),
+        ("ts", ctypes.c_float), ("freq_max", ctypes.c_float), ("fit_t0", ctypes.c_float), ("fit_t1", ctypes.c_float),
+        ("crossover", ctypes.c_float),
+    ]
+
+
+# exported symbols of include/ControllerEngine.h (checked by tests/test_abi.py)
+EXPORTS = [
+    "RunControllerEngine", "ce_config_parse", "ce_config_derive", "ce_plant_load_dir", "ce_engine_create",
+    "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times", "ce_detmath_host",
+    "ce_detmath_device",
+]
+
+_lib = None
+
+
+def _declare(lib):
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    lib.RunControllerEngine.restype = ctypes.c_int
+    lib.RunControllerEngine.argtypes = []
+    lib.ce_config_parse.argtypes = [ctypes.c_char_p, ctypes.c_char_p, P(CeConfig), ctypes.c_char_p, ctypes.c_int]
+    lib.ce_config_parse.restype = ctypes.c_int
+    lib.ce_config_derive.argtypes = [P(CeConfig), P(CeDerived), ctypes.c_char_p, ctypes.c_int]
+    lib.ce_config_derive.restype = ctypes.c_int
+    lib.ce_plant_load_dir.argtypes = [ctypes.c_char_p, P(CeConfig), ctypes.c_int, P(CePlant), ctypes.c_char_p, ctypes.c_int]
+    lib.ce_plant_load_dir.restype = ctypes.c_int
+    lib.ce_engine_create.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_engine_create.restype = vp
+    lib.ce_engine_destroy.argtypes = [vp]
+    lib.ce_engine_destroy.restype = None
+    lib.ce_evaluate.argtypes = [vp, P(CeConfig), P(CePlant), ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp,
+                                ctypes.c_char_p, ctypes.c_int]
+    lib.ce_evaluate.restype = ctypes.c_int
+    lib.ce_engine_set_diagnostics.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.ce_engine_set_diagnostics.restype = ctypes.c_int
+    lib.ce_engine_stage_times.argtypes = [vp, P(ctypes.c_double), ctypes.c_int]
+    lib.ce_engine_stage_times.restype = ctypes.c_int
+    lib.ce_detmath_host.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int]
+    lib.ce_detmath_host.restype = ctypes.c_int
+    lib.ce_detmath_device.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
+    lib.ce_detmath_device.restype = ctypes.c_int
+    for name in ("ce_batch_create", "ce_batch_destroy", "ce_batch_add", "ce_batch_run", "ce_batch_reset",
+                 "ce_server_create", "ce_server_destroy", "ce_server_submit", "ce_server_poll", "ce_server_step",
+                 "ce_server_stats"):
+        if hasattr(lib, name):
+            getattr(lib, name)  # resolved lazily by the modules that use them
+    return lib
+
+
+def lib():
+    """The HIP engine library; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libControllerEngine.so is not built: run python controllerengine_amd/build.py "
+                               "(or __graft_entry__.build())")
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def errbuf():
+    return ctypes.create_string_buffer(1024)

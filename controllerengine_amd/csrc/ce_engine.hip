@@ -1,0 +1,378 @@
+// Engine runtime: device buffers, problem upload, the stage chain, the extended C API and the
+// drop-in RunControllerEngine() file protocol (reference src/main.cu:385-602).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ce_host.h"
+#include "ce_stage.h"
+
+namespace ce {
+// kernels (ce_kernels_fp64.hip, ce_kernels_mixed.hip)
+void launch_eig_fp64(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_info_init(const float* in, double* out, size_t count, hipStream_t st);
+void launch_info_final(const double* in, float* out, size_t count, hipStream_t st);
+bool launch_mixed_chain(const ce_config& cfg, const ProblemDev* P, const StageBufs& B, hipStream_t st,
+                        hipEvent_t* ev, std::string& err);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
+void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
+void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
+}  // namespace ce
+
+using namespace ce;
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) {                                                                         \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                               ":" + std::to_string(__LINE__));                                     \
+    }                                                                                               \
+  } while (0)
+
+constexpr int N_EVENTS = 6;
+
+struct ce_engine {
+  int device = 0;
+  std::vector<uint8_t> blob;           // last uploaded problem
+  uint8_t* blob_pinned = nullptr;
+  size_t blob_pinned_cap = 0;
+  void* d_prob = nullptr;
+  size_t d_prob_cap = 0;
+  struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
+  Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail;
+  float* diag_y = nullptr;
+  float* diag_lina = nullptr;
+  float* diag_eig = nullptr;
+  float* diag_ata = nullptr;
+  float* diag_atb = nullptr;
+  hipEvent_t ev[N_EVENTS] = {};
+  bool events_ok = false;
+  hipStream_t last_stream = nullptr;
+};
+
+static void* ensure(ce_engine::Buf& b, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (b.cap < bytes) {
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    HIPCHK(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+  }
+  return b.p;
+}
+
+static void put_err(char* err, int errlen, const std::string& m) {
+  if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", m.c_str());
+}
+
+// Upload the problem blob if it differs from the one on the device (stream-ordered).
+static const ProblemDev* upload_problem(ce_engine* E, const std::vector<uint8_t>& blob, hipStream_t st) {
+  if (E->d_prob && E->blob == blob) return reinterpret_cast<const ProblemDev*>(E->d_prob);
+  // the previous blob may still be read by queued kernels: drain before reusing the buffers
+  HIPCHK(hipStreamSynchronize(st));
+  if (E->d_prob_cap < blob.size()) {
+    if (E->d_prob) HIPCHK(hipFree(E->d_prob));
+    HIPCHK(hipMalloc(&E->d_prob, blob.size()));
+    E->d_prob_cap = blob.size();
+  }
+  if (E->blob_pinned_cap < blob.size()) {
+    if (E->blob_pinned) HIPCHK(hipHostFree(E->blob_pinned));
+    HIPCHK(hipHostMalloc((void**)&E->blob_pinned, blob.size(), hipHostMallocDefault));
+    E->blob_pinned_cap = blob.size();
+  }
+  std::memcpy(E->blob_pinned, blob.data(), blob.size());
+  HIPCHK(hipMemcpyAsync(E->d_prob, E->blob_pinned, blob.size(), hipMemcpyHostToDevice, st));
+  E->blob = blob;
+  return reinterpret_cast<const ProblemDev*>(E->d_prob);
+}
+
+// The six-stage chain for one candidate block (reference src/main.cu:467-556, one round).
+static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, int precision, const float* d_gains,
+                     const float* d_info_in, float* d_info_out, double* d_info64_out, int n, hipStream_t st) {
+  if (n < 0) throw std::runtime_error("ce_evaluate: negative candidate count");
+  std::vector<uint8_t> blob;
+  std::string err;
+  if (!build_problem(cfg, plant, blob, err)) throw std::runtime_error(err);
+  const ProblemDev& H = *reinterpret_cast<const ProblemDev*>(blob.data());
+  if (n == 0) return;
+  if (!E->events_ok) {
+    for (int i = 0; i < N_EVENTS; i++) HIPCHK(hipEventCreate(&E->ev[i]));
+    E->events_ok = true;
+  }
+  const ProblemDev* P = upload_problem(E, blob, st);
+  const size_t nn = (size_t)n;
+  const int nt = H.nt, ns = H.ns, F = H.F;
+  StageBufs B{};
+  B.n = n;
+  B.gains = d_gains;
+  B.info = (double*)ensure(E->info64, nn * H.info_w * sizeof(double));
+  B.flags = (int32_t*)ensure(E->flags, nn * sizeof(int32_t));
+  B.uniq = (int32_t*)ensure(E->uniq, nn * sizeof(int32_t));
+  B.mode_re = (double*)ensure(E->mode_re, nn * MAXT * sizeof(double));
+  B.mode_im = (double*)ensure(E->mode_im, nn * MAXT * sizeof(double));
+  B.open_a = (double*)ensure(E->open_a, nn * ns * ns * sizeof(double));
+  if (precision == CE_PRECISION_FP64) B.ybuf = (double*)ensure(E->ybuf, nn * (size_t)H.fit_count * sizeof(double));
+  else B.ybuf32 = (float*)ensure(E->ybuf32, nn * (size_t)H.fit_count * sizeof(float));
+  B.mag_raw = (double*)ensure(E->mag_raw, nn * (size_t)F * 2 * sizeof(double));
+  B.bmin = (unsigned long long*)ensure(E->bmin, nn * sizeof(unsigned long long));
+  B.sweep_fail = (int32_t*)ensure(E->sfail, nn * sizeof(int32_t));
+  B.diag = (E->diag_y || E->diag_lina || E->diag_eig || E->diag_ata || E->diag_atb) ? 1 : 0;
+  B.d_y = E->diag_y; B.d_lina = E->diag_lina; B.d_eig = E->diag_eig; B.d_ata = E->diag_ata; B.d_atb = E->diag_atb;
+  if (B.diag) {
+    if (B.d_ata) HIPCHK(hipMemsetAsync(B.d_ata, 0, nn * 4 * nt * nt * sizeof(float), st));
+    if (B.d_atb) HIPCHK(hipMemsetAsync(B.d_atb, 0, nn * 2 * nt * sizeof(float), st));
+  }
+  launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
+  launch_fill_i32(B.sweep_fail, 0, nn, st);
+  launch_info_init(d_info_in, B.info, nn * H.info_w, st);
+  E->last_stream = st;
+  if (precision == CE_PRECISION_FP64) {
+    HIPCHK(hipEventRecord(E->ev[0], st));
+    launch_eig_fp64(P, B, st);
+    HIPCHK(hipEventRecord(E->ev[1], st));
+    if (!launch_sim_fp64(ns, H.nd, P, B, st)) throw std::runtime_error("no simulation kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev[2], st));
+    if (!launch_fit_fp64(nt, P, B, st)) throw std::runtime_error("no fit kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev[3], st));
+    if (!launch_sweep_fp64(ns, F, P, B, st)) throw std::runtime_error("no sweep kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev[4], st));
+  } else if (precision == CE_PRECISION_MIXED) {
+    if (!launch_mixed_chain(cfg, P, B, st, E->ev, err)) throw std::runtime_error(err);
+  } else {
+    throw std::runtime_error("ce_evaluate: unknown precision " + std::to_string(precision));
+  }
+  HIPCHK(hipGetLastError());
+  launch_info_final(B.info, d_info_out, nn * H.info_w, st);
+  if (d_info64_out)
+    HIPCHK(hipMemcpyAsync(d_info64_out, B.info, nn * H.info_w * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipEventRecord(E->ev[5], st));
+}
+
+// =================================================================================================
+// extended C API
+
+extern "C" CE_EXPORT ce_engine* ce_engine_create(int device, char* err, int errlen) {
+  try {
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (count <= 0) throw std::runtime_error("no HIP device");
+    if (device < 0 || device >= count) throw std::runtime_error("device index out of range");
+    HIPCHK(hipSetDevice(device));
+    ce_engine* E = new ce_engine();
+    E->device = device;
+    return E;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
+  if (!E) return;
+  hipDeviceSynchronize();
+  for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
+                  &E->mag_raw, &E->bmin, &E->sfail})
+    if (b->p) hipFree(b->p);
+  if (E->d_prob) hipFree(E->d_prob);
+  if (E->blob_pinned) hipHostFree(E->blob_pinned);
+  if (E->events_ok)
+    for (int i = 0; i < N_EVENTS; i++) hipEventDestroy(E->ev[i]);
+  delete E;
+}
+
+extern "C" CE_EXPORT int ce_evaluate(ce_engine* E, const ce_config* cfg, const ce_plant* plant, int precision,
+                                     const float* d_gains, const float* d_info_in, float* d_info_out,
+                                     double* d_info64_out, int n, void* stream, char* err, int errlen) {
+  try {
+    if (!E || !cfg || !plant) throw std::runtime_error("ce_evaluate: null argument");
+    HIPCHK(hipSetDevice(E->device));
+    evaluate(E, *cfg, *plant, precision, d_gains, d_info_in, d_info_out, d_info64_out, n, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+extern "C" CE_EXPORT int ce_engine_set_diagnostics(ce_engine* E, float* y, float* lina, float* eig, float* ata, float* atb) {
+  if (!E) return 1;
+  E->diag_y = y; E->diag_lina = lina; E->diag_eig = eig; E->diag_ata = ata; E->diag_atb = atb;
+  return 0;
+}
+
+extern "C" CE_EXPORT int ce_engine_stage_times(ce_engine* E, double* ms, int n_max) {
+  if (!E || !E->events_ok) return 0;
+  if (hipEventSynchronize(E->ev[N_EVENTS - 1]) != hipSuccess) return 0;
+  int k = 0;
+  for (int i = 0; i + 1 < N_EVENTS && k < n_max; i++, k++) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, E->ev[i], E->ev[i + 1]) != hipSuccess) t = -1.0f;
+    ms[k] = t;
+  }
+  return k;
+}
+
+extern "C" CE_EXPORT int ce_detmath_device(int fn, const double* x, const double* y, double* out, int n, void* stream) {
+  ce::launch_detmath(fn, x, y, out, n, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// =================================================================================================
+// Drop-in entry point: reference src/main.cu:385-602, same files, messages and return convention.
+
+static void write_file(const char* name, const void* data, size_t bytes) {
+  std::ofstream f(name, std::ios::binary);
+  if (!f) throw std::runtime_error(std::string("Cannot open ") + name);
+  f.write(reinterpret_cast<const char*>(data), (std::streamsize)bytes);
+}
+
+static void run_from_cwd() {
+  std::string err;
+  std::string text;
+  {
+    std::ifstream f("ControllerEngine.config");
+    if (!f) throw std::runtime_error("Cannot open ControllerEngine.config");
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    text = ss.str();
+  }
+  ce_config cfg;
+  if (!parse_config(text, "ControllerEngine.config", cfg, err)) throw std::runtime_error(err);
+  ce_derived d;
+  if (!derive_config(cfg, d, err)) throw std::runtime_error(err);
+  ce_plant plant;
+  if (!load_plant_dir("", cfg, 0, plant, err)) throw std::runtime_error(err);
+  {
+    // decode once up front so that undecodable instructions fail before any output is written
+    std::vector<uint8_t> blob;
+    if (!build_problem(cfg, plant, blob, err)) {
+      if (err.rfind("allmargin:", 0) != 0) throw std::runtime_error(err);
+    }
+  }
+  int precision = CE_PRECISION_FP64;
+  if (const char* p = std::getenv("CE_PRECISION")) {
+    const std::string v = p;
+    if (v == "mixed") precision = CE_PRECISION_MIXED;
+    else if (v == "fp64" || v.empty()) precision = CE_PRECISION_FP64;
+    else throw std::runtime_error("CE_PRECISION should be fp64/mixed, got `" + v + "`");
+  }
+  {
+    std::ofstream plog("params.log");
+    plog << std::scientific << std::setprecision(9);
+    for (int i = 0; i < cfg.n_params; i++) plog << "BASE_PARAMS_HOST[" << i << "] = " << plant.base_params[i] << "\n";
+    plog << "ref amplitude = " << cfg.ref_amplitude << "\n";
+  }
+  const size_t nthr = (size_t)d.n_threads;
+  const size_t gw = (size_t)d.cuda_mx_width, iw = (size_t)d.info_mx_width;
+  const size_t total = (size_t)cfg.n_rounds * nthr;
+  std::ifstream cudamx("CUDA_MX.bin", std::ios::binary);
+  if (!cudamx) throw std::runtime_error("Cannot open CUDA_MX.bin");
+  cudamx.seekg(0, std::ios::end);
+  if ((size_t)cudamx.tellg() != total * gw * 4)
+    throw std::runtime_error("CUDA_MX.bin is " + std::to_string((long long)cudamx.tellg()) + " bytes, expected " +
+                             std::to_string(total * gw * 4) + " (" + std::to_string(cfg.n_rounds) + " rounds x " +
+                             std::to_string(nthr) + " candidates)");
+  cudamx.seekg(0, std::ios::beg);
+  std::ifstream infomx("INFO_MX.bin", std::ios::binary);
+  if (!infomx) throw std::runtime_error("Cannot open INFO_MX.bin");
+  infomx.seekg(0, std::ios::end);
+  if ((size_t)infomx.tellg() != total * iw * 4)
+    throw std::runtime_error("INFO_MX.bin is " + std::to_string((long long)infomx.tellg()) + " bytes, expected " +
+                             std::to_string(total * iw * 4) + " (" + std::to_string(cfg.n_rounds) + " rounds x " +
+                             std::to_string(nthr) + " candidates)");
+  infomx.seekg(0, std::ios::beg);
+  std::ofstream infoout("INFO_MX_OUT.bin", std::ios::binary);
+  if (!infoout) throw std::runtime_error("Cannot open INFO_MX_OUT.bin");
+
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  ce_engine* E = ce_engine_create(dev, nullptr, 0);
+  if (!E) throw std::runtime_error("cannot create engine");
+  struct Guard {
+    ce_engine* e;
+    std::vector<void*> mem;
+    ~Guard() {
+      for (void* p : mem) hipFree(p);
+      ce_engine_destroy(e);
+    }
+  } guard{E, {}};
+  auto dalloc = [&](size_t bytes) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes ? bytes : 4));
+    guard.mem.push_back(p);
+    return p;
+  };
+  const int nt = d.total_dim;
+  float* d_g = (float*)dalloc(nthr * gw * 4);
+  float* d_in = (float*)dalloc(nthr * iw * 4);
+  float* d_out = (float*)dalloc(nthr * iw * 4);
+  std::vector<float> h_g(nthr * gw), h_info(nthr * iw);
+  float *dy = nullptr, *dl = nullptr, *de = nullptr, *da = nullptr, *db = nullptr;
+  if (cfg.dump_diagnostics) {
+    dy = (float*)dalloc(nthr * (size_t)cfg.time * 4);
+    dl = (float*)dalloc(nthr * (size_t)nt * nt * 4);
+    de = (float*)dalloc(nthr * (size_t)2 * nt * 4);
+    da = (float*)dalloc(nthr * (size_t)4 * nt * nt * 4);
+    db = (float*)dalloc(nthr * (size_t)2 * nt * 4);
+    ce_engine_set_diagnostics(E, dy, dl, de, da, db);
+  }
+  hipStream_t st = nullptr;
+  for (int round = 0; round < cfg.n_rounds; round++) {
+    if (cfg.serial && round > 0)
+      if (!load_plant_dir("", cfg, round, plant, err)) throw std::runtime_error(err);
+    cudamx.read(reinterpret_cast<char*>(h_g.data()), (std::streamsize)(h_g.size() * 4));
+    if (!cudamx)
+      throw std::runtime_error("Cannot read CUDA_MX.bin round " + std::to_string(round) + " (intended " +
+                               std::to_string(h_g.size() * 4) + "-byte allocation)");
+    infomx.read(reinterpret_cast<char*>(h_info.data()), (std::streamsize)(h_info.size() * 4));
+    if (!infomx)
+      throw std::runtime_error("Cannot read INFO_MX.bin round " + std::to_string(round) + " (intended " +
+                               std::to_string(h_info.size() * 4) + "-byte allocation)");
+    HIPCHK(hipMemcpy(d_g, h_g.data(), h_g.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_in, h_info.data(), h_info.size() * 4, hipMemcpyHostToDevice));
+    evaluate(E, cfg, plant, precision, d_g, d_in, d_out, nullptr, (int)nthr, st);
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(h_info.data(), d_out, h_info.size() * 4, hipMemcpyDeviceToHost));
+    if (cfg.dump_diagnostics) {
+      std::vector<float> tmp;
+      auto dump = [&](const char* name, const float* dp, size_t count) {
+        tmp.resize(count);
+        HIPCHK(hipMemcpy(tmp.data(), dp, count * 4, hipMemcpyDeviceToHost));
+        write_file(name, tmp.data(), count * 4);
+      };
+      dump("YPLANT_OUT.bin", dy, nthr * (size_t)cfg.time);
+      dump("LINA_MX_OUT.bin", dl, nthr * (size_t)nt * nt);
+      dump("EIGEN_MX_OUT.bin", de, nthr * (size_t)2 * nt);
+      dump("ATA_MX_OUT.bin", da, nthr * (size_t)4 * nt * nt);
+      dump("ATB_MX_OUT.bin", db, nthr * (size_t)2 * nt);
+    }
+    infoout.write(reinterpret_cast<const char*>(h_info.data()), (std::streamsize)(h_info.size() * 4));
+    infoout.flush();
+  }
+}
+
+extern "C" CE_EXPORT int RunControllerEngine(void) {
+  try {
+    run_from_cwd();
+    return 0;
+  } catch (const std::exception& e) {
+    std::ofstream elog("error.log", std::ios::app);
+    elog << "ControllerEngine error: " << e.what() << std::endl;
+    return 1;
+  }
+}

@@ -1,0 +1,37 @@
+// Per-launch stage buffers shared by the engine and the kernels.
+#pragma once
+
+#include "ce_common.h"
+
+namespace ce {
+
+enum : int32_t {
+  F_UNSTABLE = 1,          // some closed-loop eigenvalue has |lambda| > 1 (reference src/modalsetup.cu:26)
+  F_NOMINAL_UNSTABLE = 2,  // nominal loop of the margins stage unstable (reference src/allmargin.cu:61-87)
+  F_NOMINAL_SKIPPED = 4    // nominal check not run (candidate already at or above unstable_err)
+};
+
+struct StageBufs {
+  int32_t n;                    // candidates in this launch
+  int32_t diag;                 // nonzero: fill the diagnostic dumps
+  const float* gains;           // n x cuda_w
+  double* info;                 // n x info_w, double copy of the INFO rows
+  int32_t* flags;               // n
+  int32_t* uniq;                // n, unique modes
+  double* mode_re;              // MAXT x n (SoA)
+  double* mode_im;              // MAXT x n
+  double* open_a;               // n x ns*ns (LIN_A top-left block, per candidate contiguous)
+  double* ybuf;                 // fit_count x n (SoA), FP64 path
+  float* ybuf32;                // fit_count x n (SoA), mixed path
+  double* mag_raw;              // n x F x 2: |loop| and arg(loop) per frequency
+  unsigned long long* bmin;     // n: bit pattern of min b(w) (non-negative doubles order as integers)
+  int32_t* sweep_fail;          // n
+  // diagnostics in the reference dump layout (float, candidate-major), NULL when not requested
+  float* d_y;
+  float* d_lina;
+  float* d_eig;
+  float* d_ata;
+  float* d_atb;
+};
+
+}  // namespace ce

@@ -1,0 +1,90 @@
+"""Python host mirror of the engine: device-resident evaluation through the C ABI.
+
+torch provides device memory and the stream handle only; every stage runs in the HIP kernels of
+libControllerEngine.so.  There is no CPU fallback: without the library or a GPU the calls raise.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+def _dev_ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Engine:
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("Engine needs a HIP device")
+        self.torch = torch
+        self.device = device
+        err = _abi.errbuf()
+        self.h = _abi.lib().ce_engine_create(int(device), err, len(err))
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self._diag = None
+
+    def close(self):
+        if self.h:
+            _abi.lib().ce_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream_handle(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def to_device(self, arr, dtype=None):
+        t = self.torch.from_numpy(np.ascontiguousarray(arr))
+        return t.to(f"cuda:{self.device}")
+
+    def set_diagnostics(self, n, cfg, enable=True):
+        torch = self.torch
+        if not enable:
+            _abi.lib().ce_engine_set_diagnostics(self.h, None, None, None, None, None)
+            self._diag = None
+            return None
+        nt = cfg.statespace_dim + cfg.discrete_dim
+        dev = f"cuda:{self.device}"
+        d = dict(y=torch.zeros((n, cfg.time), dtype=torch.float32, device=dev),
+                 lina=torch.zeros((n, nt * nt), dtype=torch.float32, device=dev),
+                 eig=torch.zeros((n, 2 * nt), dtype=torch.float32, device=dev),
+                 ata=torch.zeros((n, 4 * nt * nt), dtype=torch.float32, device=dev),
+                 atb=torch.zeros((n, 2 * nt), dtype=torch.float32, device=dev))
+        _abi.lib().ce_engine_set_diagnostics(self.h, _dev_ptr(d["y"]), _dev_ptr(d["lina"]), _dev_ptr(d["eig"]),
+                                             _dev_ptr(d["ata"]), _dev_ptr(d["atb"]))
+        self._diag = d
+        return d
+
+    def evaluate_device(self, cfg, plant, precision, d_gains, d_info_in, d_info_out, d_info64=None):
+        """Enqueue the chain on torch's current stream; all tensors on this device."""
+        n = int(d_gains.shape[0])
+        err = _abi.errbuf()
+        rc = _abi.lib().ce_evaluate(self.h, ctypes.byref(cfg), ctypes.byref(plant), int(precision),
+                                    _dev_ptr(d_gains), _dev_ptr(d_info_in), _dev_ptr(d_info_out),
+                                    _dev_ptr(d_info64), n, self.stream_handle(), err, len(err))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+
+    def evaluate(self, problem, precision=_abi.PRECISION_FP64, rnd=0, gains=None, info=None, want64=True):
+        """Host convenience: copy rows in, run, copy rows out (float32 rows and, optionally, double rows)."""
+        torch = self.torch
+        g = self.to_device(np.asarray(problem.gains if gains is None else gains, dtype=np.float32))
+        i_in = self.to_device(np.asarray(problem.info if info is None else info, dtype=np.float32))
+        i_out = torch.empty_like(i_in)
+        i64 = torch.empty(i_in.shape, dtype=torch.float64, device=i_in.device) if want64 else None
+        self.evaluate_device(problem.cfg_struct(), problem.plant_struct(rnd), precision, g, i_in, i_out, i64)
+        torch.cuda.synchronize(self.device)
+        return i_out.cpu().numpy(), (i64.cpu().numpy() if want64 else None)
+
+    def stage_times(self):
+        buf = (ctypes.c_double * 8)()
+        k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)
+        return [buf[i] for i in range(k)]

@@ -1,0 +1,152 @@
+/*
+ * ControllerEngine for AMD Instinct MI355X (gfx950): public C ABI.
+ *
+ * Two layers are exported from libControllerEngine.so (also shipped as ControllerEngine.so):
+ *
+ *  1. The drop-in entry point, RunControllerEngine().  It replaces the reference library's only export
+ *     (reference src/ControllerEngine.h:11, defined at src/main.cu:592-602): same name, no arguments,
+ *     same file protocol in the current working directory (ControllerEngine.config, STATESPACE_*.bin,
+ *     DISCRETE_*.bin, BASE_PARAMS.bin, DC/U/NL_INFO.bin, CUDA_MX.bin, INFO_MX.bin in;
+ *     INFO_MX_OUT.bin, params.log and the optional stage dumps out), same return convention
+ *     (0 = success, 1 = failure with the message appended to error.log).
+ *     The arithmetic mode is chosen by the environment variable CE_PRECISION = fp64 (default) | mixed,
+ *     because the reference ABI takes no arguments and its config parser rejects unknown keys
+ *     (src/config.cu:239).
+ *
+ *  2. An extended API (no counterpart in the reference) for callers that keep inputs on the device:
+ *     config parsing, plant loading, device-resident evaluation of a candidate block, batched
+ *     evaluation of many problems in one launch chain, and a continuous-batching server.
+ *
+ * All pointers are plain pointers; device pointers are HIP device addresses, streams are hipStream_t
+ * passed as void*.  Functions return 0 on success and nonzero on failure; on failure a message is
+ * written to err (errlen bytes, NUL-terminated) when err is not NULL.
+ */
+#ifndef CONTROLLERENGINE_H
+#define CONTROLLERENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(_WIN32)
+#define CE_EXPORT __declspec(dllexport)
+#else
+#define CE_EXPORT __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Drop-in entry point. Reference: src/ControllerEngine.h:11, src/main.cu:592-602. */
+CE_EXPORT int RunControllerEngine(void);
+
+/* Compile-time limits. Reference: src/main.cu:43-52. */
+enum {
+  CE_MAX_STATESPACE_DIM = 8,
+  CE_MAX_DISCRETE_DIM = 8,
+  CE_MAX_NONLIN_DIM = 4,
+  CE_MAX_TOTAL_DIM = 16,
+  CE_MAX_NL_DIM = 9,
+  CE_MAX_DC_INSTRUCTIONS = 4,
+  CE_MAX_U_INSTRUCTIONS = 4,
+  CE_MAX_NL_INSTRUCTIONS = 8,
+  CE_MAX_PARAMS = 34
+};
+
+/* Arithmetic modes. */
+enum {
+  CE_PRECISION_FP64 = 0,  /* every stage in double */
+  CE_PRECISION_MIXED = 1  /* float simulation, fit accumulation and sweep; double eigensolve,
+                             normal-equation solve and crossover refinement */
+};
+
+/* The parsed ControllerEngine.config. Field meaning follows reference src/config.cu:5-27; every
+ * key, default and validation rule of load_config (src/config.cu:106-244) applies. */
+typedef struct ce_config {
+  int32_t statespace_dim, discrete_dim, nonlin_dim, n_params, input_size;
+  int32_t dc_instructions, u_instructions;
+  int32_t nl_instructions[CE_MAX_NONLIN_DIM];
+  int32_t n_cores, n_rounds;
+  int32_t serial;            /* mode = serial */
+  int32_t time;              /* samples per run */
+  float duration;
+  double ref_step_time;
+  float ref_amplitude;
+  float fit_start, fit_end, steady_start;
+  float bandwidth, damping, overshoot, steady_state_error, crossover_coeff;
+  float target_gm, target_pm, target_b;
+  float eig_fail_err, unstable_err;
+  int32_t freq;
+  float freq_min;
+  int32_t dump_diagnostics;
+} ce_config;
+
+/* One round's plant and observer model, compact row-major, exactly the .bin contents
+ * (reference src/main.cu:341-400 reads them). */
+typedef struct ce_plant {
+  float ss_a[CE_MAX_STATESPACE_DIM * CE_MAX_STATESPACE_DIM];    /* STATESPACE_A    ns x ns */
+  float ss_a_i[CE_MAX_STATESPACE_DIM * CE_MAX_STATESPACE_DIM];  /* STATESPACE_A_I  ns x ns */
+  float ss_b[CE_MAX_STATESPACE_DIM];                            /* STATESPACE_B    ns */
+  float ss_c[CE_MAX_STATESPACE_DIM];                            /* STATESPACE_C    ns */
+  float ss_d;                                                   /* STATESPACE_D    1 */
+  float d_a[CE_MAX_DISCRETE_DIM * CE_MAX_DISCRETE_DIM];         /* DISCRETE_A      nd x nd */
+  float d_a_i[CE_MAX_DISCRETE_DIM * CE_MAX_DISCRETE_DIM];       /* DISCRETE_A_I    nd x nd */
+  float d_bu[CE_MAX_DISCRETE_DIM];                              /* DISCRETE_BU     nd */
+  float d_bw[CE_MAX_DISCRETE_DIM * CE_MAX_NONLIN_DIM];          /* DISCRETE_BW     nd x nonlin_dim */
+  float d_c[CE_MAX_DISCRETE_DIM];                               /* DISCRETE_C      nd */
+  float base_params[CE_MAX_PARAMS];                             /* BASE_PARAMS     n_params */
+  int32_t dc_info[CE_MAX_DC_INSTRUCTIONS * CE_MAX_DISCRETE_DIM];               /* dc_instructions x nd */
+  int32_t u_info[CE_MAX_U_INSTRUCTIONS * CE_MAX_STATESPACE_DIM];               /* u_instructions x ns */
+  int32_t nl_info[CE_MAX_NONLIN_DIM * CE_MAX_NL_INSTRUCTIONS * CE_MAX_NL_DIM]; /* sum(nl) x (nd+1) */
+} ce_plant;
+
+/* Quantities derive() computes from the config (reference src/config.cu:40-78, 93-104), in the
+ * reference's float32 arithmetic so that sample counts and indices agree. */
+typedef struct ce_derived {
+  int32_t n_threads, total_dim, cuda_mx_width, info_mx_width;
+  int32_t info_scoring, info_freqresp, info_allmargin, info_modal;
+  int32_t fit_start_id, steady_start_id, fit_count; /* fit_count = samples the fit loop visits */
+  float ts, freq_max, fit_t0, fit_t1, crossover;
+} ce_derived;
+
+/* Parse config text (file name used in messages). Reference: src/config.cu:106-244. */
+CE_EXPORT int ce_config_parse(const char* text, const char* filename, ce_config* out, char* err, int errlen);
+CE_EXPORT int ce_config_derive(const ce_config* cfg, ce_derived* out, char* err, int errlen);
+/* Read round `round` of the plant files from directory dir (exact byte sizes enforced).
+ * Reference: src/main.cu:293-315 (readbin), 341-400 (upload_plant, init64KB). */
+CE_EXPORT int ce_plant_load_dir(const char* dir, const ce_config* cfg, int round, ce_plant* out, char* err, int errlen);
+
+/* ---------------------------------------------------------------------------------------------
+ * Device evaluation of one candidate block (the six-stage chain of reference src/main.cu:467-558
+ * for one round).  gains: n x (2*statespace_dim+1) floats, info_in/info_out: n x info_mx_width
+ * floats, all device pointers.  info64_out (optional, may be NULL) receives the same rows in double.
+ * Work is enqueued on `stream`; the call returns after enqueueing unless it has to report an error
+ * found on the host.  An engine holds device scratch sized by its largest call so far. */
+typedef struct ce_engine ce_engine;
+
+CE_EXPORT ce_engine* ce_engine_create(int device, char* err, int errlen);
+CE_EXPORT void ce_engine_destroy(ce_engine* eng);
+CE_EXPORT int ce_evaluate(ce_engine* eng, const ce_config* cfg, const ce_plant* plant, int precision,
+                          const float* d_gains, const float* d_info_in, float* d_info_out, double* d_info64_out,
+                          int n, void* stream, char* err, int errlen);
+
+/* Optional per-stage device outputs of the last ce_evaluate call (the reference's dump_diagnostics
+ * files, src/main.cu:483-547).  Sizes per candidate: yplant time, lina nt*nt, eigen 2*nt,
+ * ata (2nt)^2 (lower triangle, row stride 2nt), atb 2nt.  All float, candidate-major like the
+ * reference dumps. Any pointer may be NULL. */
+CE_EXPORT int ce_engine_set_diagnostics(ce_engine* eng, float* d_yplant, float* d_lina, float* d_eigen,
+                                        float* d_ata, float* d_atb);
+
+/* Per-stage timings of the last ce_evaluate (milliseconds, from HIP events on its stream). */
+CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
+
+/* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
+ * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
+CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);
+CE_EXPORT int ce_detmath_device(int fn, const double* d_x, const double* d_y, double* d_out, int n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CONTROLLERENGINE_H */
