@@ -27,8 +27,16 @@ FLAGS = [
 ]
 
 
-def _compile(src):
+def _headers_mtime():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hs.append(os.path.join(os.path.dirname(HERE), "include", "ControllerEngine.h"))
+    return max(os.path.getmtime(h) for h in hs if os.path.exists(h))
+
+
+def _compile(src, force=True):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), _headers_mtime()):
+        return obj
     cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
@@ -38,12 +46,12 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, force=True):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
     workers = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     with ThreadPoolExecutor(workers) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -58,5 +66,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(force="--incremental" not in sys.argv)
     sys.exit(0)

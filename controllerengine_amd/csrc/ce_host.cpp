@@ -30,6 +30,20 @@ std::string fmt_num(double v) {
   return buf;
 }
 
+// Shortest representation that reads back to the same float (std::format("{}") of a float).
+std::string fmt_flt(float v) {
+  char buf[64];
+  if (v == std::floor(v) && std::fabs(v) < 1e16f) {
+    std::snprintf(buf, sizeof buf, "%.0f", (double)v);
+    return buf;
+  }
+  for (int p = 1; p <= 9; p++) {
+    std::snprintf(buf, sizeof buf, "%.*g", p, (double)v);
+    if (std::strtof(buf, nullptr) == v) break;
+  }
+  return buf;
+}
+
 static std::string trim(const std::string& s) {
   const size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
   return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
@@ -150,7 +164,7 @@ bool parse_config(const std::string& text, const std::string& filename, ce_confi
     in_range("fit_end", c.fit_end, 0.0, 1.0);
     in_range("steady_start", c.steady_start, 0.0, 1.0);
     if (!(c.fit_start < c.fit_end))
-      fail(filename + ": fit_start (" + fmt_num(c.fit_start) + ") must be below fit_end (" + fmt_num(c.fit_end) + ")");
+      fail(filename + ": fit_start (" + fmt_flt(c.fit_start) + ") must be below fit_end (" + fmt_flt(c.fit_end) + ")");
     if ((int)(c.steady_start * c.time) >= c.time) fail(filename + ": steady_start leaves no samples to average");
     c.bandwidth = get_float("bandwidth");
     c.damping = get_float("damping");
@@ -184,8 +198,8 @@ bool parse_config(const std::string& text, const std::string& filename, ce_confi
       return false;
     }
     if (!(d.crossover > 0.0f && d.crossover < d.freq_max))
-      fail(filename + ": crossover " + fmt_num(d.crossover) + " rad/s (from bandwidth, damping, crossover_coeff) must be in (0, Nyquist = " +
-           fmt_num(d.freq_max) + ")");
+      fail(filename + ": crossover " + fmt_flt(d.crossover) + " rad/s (from bandwidth, damping, crossover_coeff) must be in (0, Nyquist = " +
+           fmt_flt(d.freq_max) + ")");
   } catch (const Fail&) {
     return false;
   }
@@ -489,7 +503,7 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
     gr = gr + P.pd;
     const double pmag = cedm::hypot(gr, gi);
     if (!ok || !(pmag > 0.0) || !cedm::isfinite_(pmag)) {
-      err = "allmargin: plant has no finite nonzero gain at crossover = " + fmt_num(d.crossover) + " rad/s";
+      err = "allmargin: plant has no finite nonzero gain at crossover = " + fmt_flt(d.crossover) + " rad/s";
       return false;
     }
     P.alpha = 1.0 / pmag;

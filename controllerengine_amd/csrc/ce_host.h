@@ -9,6 +9,7 @@
 namespace ce {
 
 std::string fmt_num(double v);
+std::string fmt_flt(float v);
 bool parse_config(const std::string& text, const std::string& filename, ce_config& c, std::string& err);
 bool derive_config(const ce_config& c, ce_derived& d, std::string& err);
 bool read_bin(const std::string& path, const std::string& name, size_t nbytes, void* dst, int slice, int slices,
