@@ -164,22 +164,24 @@ CE_HD double reduce_pio2(double x, int& quadrant) {
   double r = fma_(-kd, PIO2_1, x);
   r = fma_(-kd, PIO2_2, r);
   r = fma_(-kd, PIO2_3, r);
-  const int64_t k = (int64_t)kd;
-  quadrant = (int)(k & 3);
+  const double kq = kd - 4.0 * floor_(kd * 0.25);   // kd mod 4, exact for |kd| < 2^53
+  quadrant = isfinite_(kq) ? (int)kq : 0;
   return r;
 }
 
 CE_HD void sincos(double x, double& s, double& c) {
-  if (!isfinite_(x)) { s = qnan(); c = qnan(); return; }
   int q;
   const double r = reduce_pio2(x, q);
   const double sr = sin_poly(r), cr = cos_poly(r);
-  switch (q) {
-    case 0: s = sr; c = cr; break;
-    case 1: s = cr; c = -sr; break;
-    case 2: s = -sr; c = -cr; break;
-    default: s = -cr; c = sr; break;
-  }
+  // quadrant map without lane-divergent branches: swap on odd quadrants, then signs
+  const bool odd = (q & 1) != 0;
+  double ss = odd ? cr : sr;
+  double cc = odd ? sr : cr;
+  ss = (q & 2) ? -ss : ss;
+  cc = ((q + 1) & 2) ? -cc : cc;
+  const bool fin = isfinite_(x);
+  s = fin ? ss : qnan();
+  c = fin ? cc : qnan();
 }
 
 CE_HD double sin(double x) { double s, c; sincos(x, s, c); return s; }
@@ -187,32 +189,20 @@ CE_HD double cos(double x) { double s, c; sincos(x, s, c); return c; }
 
 // ---------------------------------------------------------------------------------------------
 // atan(j/8) rounded to double (hi) and the residual (lo), j = 0..8
-CE_HD double atan_tab_hi(int j) {
-  switch (j) {
-    case 0: return 0.0;
-    case 1: return 0x1.fd5ba9aac2f6ep-4;
-    case 2: return 0x1.f5b75f92c80ddp-3;
-    case 3: return 0x1.6f61941e4def1p-2;
-    case 4: return 0x1.dac670561bb4fp-2;
-    case 5: return 0x1.1e00babdefeb4p-1;
-    case 6: return 0x1.4978fa3269ee1p-1;
-    case 7: return 0x1.700a7c5784634p-1;
-    default: return 0x1.921fb54442d18p-1;
-  }
-}
-CE_HD double atan_tab_lo(int j) {
-  switch (j) {
-    case 0: return 0.0;
-    case 1: return -0x1.cd37686760c17p-59;
-    case 2: return 0x1.8ab6e3cf7afbdp-57;
-    case 3: return -0x1.c63aae6f6e918p-56;
-    case 4: return 0x1.a2b7f222f65e2p-56;
-    case 5: return -0x1.928df287a668fp-58;
-    case 6: return 0x1.2419a87f2a458p-56;
-    case 7: return -0x1.8c34d25aadef6p-56;
-    default: return 0x1.1a62633145c07p-55;
-  }
-}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __constant__ static const double ATAN_HI[9] = {
+#else
+static const double ATAN_HI[9] = {
+#endif
+    0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2, 0x1.dac670561bb4fp-2,
+    0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1, 0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1};
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __constant__ static const double ATAN_LO[9] = {
+#else
+static const double ATAN_LO[9] = {
+#endif
+    0.0, -0x1.cd37686760c17p-59, 0x1.8ab6e3cf7afbdp-57, -0x1.c63aae6f6e918p-56, 0x1.a2b7f222f65e2p-56,
+    -0x1.928df287a668fp-58, 0x1.2419a87f2a458p-56, -0x1.8c34d25aadef6p-56, 0x1.1a62633145c07p-55};
 
 // atan(t) for t in [0, 1]
 CE_HD double atan01(double t) {
@@ -231,33 +221,30 @@ CE_HD double atan01(double t) {
   p = fma_(p, z, -0x1.5555555555555p-2);    // -1/3
   p = p * z;
   const double series = fma_(p, u, u);
-  return atan_tab_hi(j) + (atan_tab_lo(j) + series);
+  return ATAN_HI[j] + (ATAN_LO[j] + series);
 }
 
-// C99 atan2 semantics for zeros, infinities and NaN.
+// C99 atan2 semantics for zeros, infinities and NaN.  The finite nonzero path is evaluated for
+// every lane (one atan01 call, selects for the octant), special cases are patched in afterwards.
 CE_HD double atan2(double y, double x) {
-  if (isnan_(x) || isnan_(y)) return qnan();
   const bool yneg = (d_to_bits(y) >> 63) != 0;
   const bool xneg = (d_to_bits(x) >> 63) != 0;
   const double ax = fabs_(x), ay = fabs_(y);
-  double r;
-  if (ay == 0.0) {
-    r = xneg ? PI_HI : 0.0;
-  } else if (ax == 0.0) {
-    r = PIO2_HI;
-  } else if (!isfinite_(ax) || !isfinite_(ay)) {
+  const bool swap = ay > ax;
+  const double num = swap ? ax : ay, den = swap ? ay : ax;
+  double t = num / den;
+  t = (t == t && t <= 1.0) ? t : 0.0;  // keeps atan01's table index in range for 0/0, inf/inf
+  double a = atan01(t);
+  a = swap ? PIO2_HI - (a - PIO2_LO) : a;
+  double r = xneg ? PI_HI - (a - PI_LO) : a;
+  if (ay == 0.0) r = xneg ? PI_HI : 0.0;
+  else if (ax == 0.0) r = PIO2_HI;
+  else if (!isfinite_(ax) || !isfinite_(ay)) {
     if (!isfinite_(ax) && !isfinite_(ay)) r = xneg ? 3.0 * (PIO2_HI * 0.5) : PIO2_HI * 0.5;
     else if (!isfinite_(ax)) r = xneg ? PI_HI : 0.0;
     else r = PIO2_HI;
-  } else {
-    double a;
-    if (ay <= ax) {
-      a = atan01(ay / ax);
-    } else {
-      a = PIO2_HI - (atan01(ax / ay) - PIO2_LO);
-    }
-    r = xneg ? PI_HI - (a - PI_LO) : a;
   }
+  if (isnan_(x) || isnan_(y)) return qnan();
   return yneg ? -r : r;
 }
 

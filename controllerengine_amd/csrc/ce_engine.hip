@@ -26,6 +26,7 @@ void launch_info_final(const double* in, float* out, size_t count, hipStream_t s
 bool launch_mixed_chain(const ce_config& cfg, const ProblemDev* P, const StageBufs& B, hipStream_t st,
                         hipEvent_t* ev, std::string& err);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
+void launch_compact(const StageBufs& B, int all, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
 }  // namespace ce
@@ -54,7 +55,7 @@ struct ce_engine {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail;
+  Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
   float* diag_y = nullptr;
   float* diag_lina = nullptr;
   float* diag_eig = nullptr;
@@ -131,6 +132,8 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   B.mag_raw = (double*)ensure(E->mag_raw, nn * (size_t)F * 2 * sizeof(double));
   B.bmin = (unsigned long long*)ensure(E->bmin, nn * sizeof(unsigned long long));
   B.sweep_fail = (int32_t*)ensure(E->sfail, nn * sizeof(int32_t));
+  B.list = (int32_t*)ensure(E->list, nn * sizeof(int32_t));
+  B.count = (int32_t*)ensure(E->count, sizeof(int32_t));
   B.diag = (E->diag_y || E->diag_lina || E->diag_eig || E->diag_ata || E->diag_atb) ? 1 : 0;
   B.d_y = E->diag_y; B.d_lina = E->diag_lina; B.d_eig = E->diag_eig; B.d_ata = E->diag_ata; B.d_atb = E->diag_atb;
   if (B.diag) {
@@ -139,11 +142,13 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   }
   launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
   launch_fill_i32(B.sweep_fail, 0, nn, st);
+  launch_fill_i32(B.count, 0, 1, st);
   launch_info_init(d_info_in, B.info, nn * H.info_w, st);
   E->last_stream = st;
   if (precision == CE_PRECISION_FP64) {
     HIPCHK(hipEventRecord(E->ev[0], st));
     launch_eig_fp64(P, B, st);
+    launch_compact(B, B.d_y != nullptr, st);
     HIPCHK(hipEventRecord(E->ev[1], st));
     if (!launch_sim_fp64(ns, H.nd, P, B, st)) throw std::runtime_error("no simulation kernel for these dimensions");
     HIPCHK(hipEventRecord(E->ev[2], st));
@@ -186,7 +191,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (!E) return;
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
-                  &E->mag_raw, &E->bmin, &E->sfail})
+                  &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

@@ -35,6 +35,22 @@ __global__ __launch_bounds__(256) void k_info_final(const double* __restrict__ i
   if (i < count) out[i] = (float)in[i];
 }
 
+// Compaction: the candidates whose trajectory is needed (linearly stable ones, or all of them when
+// the trajectory dump is requested) get consecutive slots; one atomic per wave.
+__global__ __launch_bounds__(64) void k_compact(StageBufs B, int all) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  const bool pred = c < B.n && (all || (B.flags[c] & F_UNSTABLE) == 0);
+  const unsigned long long ballot = __ballot(pred);
+  const int cnt = __popcll(ballot);
+  int base = 0;
+  if (threadIdx.x == __ffsll((long long)ballot) - 1 && cnt > 0) base = atomicAdd(B.count, cnt);
+  base = __shfl(base, __ffsll((long long)ballot) - 1);
+  if (pred) {
+    const unsigned long long below = ballot & ((1ull << threadIdx.x) - 1ull);
+    B.list[base + __popcll(below)] = c;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Eigen stage: LIN_A (reference src/eigensetup.cu:16-56), its eigenvalues, the continuous-time
 // modes with the adjacent-duplicate rule (src/modalsetup.cu:8-47), the unstable flag
@@ -165,10 +181,10 @@ __device__ __forceinline__ double interp_dev(const Ins* __restrict__ ops, int co
 // Closed-loop simulation, reference src/discrete_solves.cu:12-79, in double without hi/lo pairs.
 template <int NS, int ND>
 __global__ __launch_bounds__(64) void k_sim_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= B.n) return;
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= *B.count) return;
+  const int c = B.list[idx];
   const bool unstable = (B.flags[c] & F_UNSTABLE) != 0;
-  if (unstable && !B.diag) return;
   const int n = B.n;
   const float* g = B.gains + (size_t)c * P->cuda_w;
   double K[NS], L[NS];
@@ -185,7 +201,7 @@ __global__ __launch_bounds__(64) void k_sim_fp64(const ProblemDev* __restrict__ 
   const int dc_n = P->dc_n, u_n = P->u_n;
   const double amp = P->ref_amp, sd = P->sd;
   double ysum = 0.0, ymax = -cedm::inf(), ymin = cedm::inf();
-  double* ybuf = B.ybuf + c;
+  double* ybuf = B.ybuf + idx;
   float* dy = (B.diag && B.d_y) ? B.d_y + (size_t)c * T : nullptr;
   for (int t = 0; t < T; t++) {
     const double y = interp_dev<ND, CE_MAX_DC_INSTRUCTIONS, false>(P->dc_ops, dc_n, xp, 0.0);
@@ -278,13 +294,15 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
 
   const int lane = threadIdx.x;
   const int slot = lane / G, g = lane % G;
-  const int c = blockIdx.x * CPW + slot;
+  const int idx = blockIdx.x * CPW + slot;
   const int n = B.n;
   const int nt = P->nt, nc2 = 2 * nt;
   bool active = false;
   int u = 0;
+  int c = 0;
   double* row = nullptr;
-  if (c < n) {
+  if (idx < *B.count) {
+    c = B.list[idx];
     row = B.info + (size_t)c * P->info_w;
     const bool stable = (B.flags[c] & F_UNSTABLE) == 0;
     active = stable && (row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)));
@@ -333,7 +351,7 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
 
   const double yss = active ? row[P->info_freqresp + 3] : 0.0;
   const int f0 = P->fit_start_id, cnt = P->fit_count;
-  const double* ycol = B.ybuf + (c < n ? c : 0);
+  const double* ycol = B.ybuf + (active ? idx : 0);
   const int rbase = rb * 8, cbase = cb * 8 + coff;
   for (int tb = 0; tb < cnt; tb += TB) {
     // stage basis values of TB samples
@@ -785,6 +803,10 @@ bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hi
 
 void launch_eig_fp64(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_eig_fp64, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+}
+
+void launch_compact(const StageBufs& B, int all, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact, dim3((B.n + 63) / 64), dim3(64), 0, st, B, all);
 }
 
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {

@@ -26,6 +26,8 @@ struct StageBufs {
   double* mag_raw;              // n x F x 2: |loop| and arg(loop) per frequency
   unsigned long long* bmin;     // n: bit pattern of min b(w) (non-negative doubles order as integers)
   int32_t* sweep_fail;          // n
+  int32_t* list;                // n: candidates that reach the simulation (compacted)
+  int32_t* count;               // 1: entries in list
   // diagnostics in the reference dump layout (float, candidate-major), NULL when not requested
   float* d_y;
   float* d_lina;
