@@ -67,7 +67,8 @@ class CeDerived(ctypes.Structure):
 # exported symbols of include/ControllerEngine.h (checked by tests/test_abi.py)
 EXPORTS = [
     "RunControllerEngine", "ce_config_parse", "ce_config_derive", "ce_plant_load_dir", "ce_engine_create",
-    "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times", "ce_detmath_host",
+    "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times",
+    "ce_engine_last_sim_jit", "ce_detmath_host",
     "ce_detmath_device",
 ]
 
@@ -96,6 +97,8 @@ def _declare(lib):
     lib.ce_engine_set_diagnostics.restype = ctypes.c_int
     lib.ce_engine_stage_times.argtypes = [vp, P(ctypes.c_double), ctypes.c_int]
     lib.ce_engine_stage_times.restype = ctypes.c_int
+    lib.ce_engine_last_sim_jit.argtypes = [vp]
+    lib.ce_engine_last_sim_jit.restype = ctypes.c_int
     lib.ce_detmath_host.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int]
     lib.ce_detmath_host.restype = ctypes.c_int
     lib.ce_detmath_device.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "ce_host.h"
+#include "ce_jit.h"
 #include "ce_stage.h"
 
 namespace ce {
@@ -64,6 +65,8 @@ struct ce_engine {
   hipEvent_t ev[N_EVENTS] = {};
   bool events_ok = false;
   hipStream_t last_stream = nullptr;
+  JitCache* jit = nullptr;
+  int last_sim_jit = 0;  // 1 when the last simulation ran the specialised kernel
 };
 
 static void* ensure(ce_engine::Buf& b, size_t bytes) {
@@ -150,7 +153,21 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     launch_eig_fp64(P, B, st);
     launch_compact(B, B.d_y != nullptr, st);
     HIPCHK(hipEventRecord(E->ev[1], st));
-    if (!launch_sim_fp64(ns, H.nd, P, B, st)) throw std::runtime_error("no simulation kernel for these dimensions");
+    void* fn = nullptr;
+    E->last_sim_jit = 0;
+    if (jit_enabled()) {
+      if (!E->jit) E->jit = jit_cache_create();
+      std::string jerr;
+      if (jit_get(E->jit, gen_sim_source(H, false, 0), &fn, jerr)) {
+        if (!jit_launch_sim(fn, (n + 63) / 64, st, B.gains, B.flags, B.list, B.count, B.ybuf, B.info, B.d_y, B.mode_re,
+                            B.mode_im, B.uniq, nullptr, n))
+          throw std::runtime_error("launch of the specialised simulation kernel failed");
+        E->last_sim_jit = 1;
+      } else {
+        fn = nullptr;
+      }
+    }
+    if (!fn && !launch_sim_fp64(ns, H.nd, P, B, st)) throw std::runtime_error("no simulation kernel for these dimensions");
     HIPCHK(hipEventRecord(E->ev[2], st));
     if (!launch_fit_fp64(nt, P, B, st)) throw std::runtime_error("no fit kernel for these dimensions");
     HIPCHK(hipEventRecord(E->ev[3], st));
@@ -195,6 +212,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);
+  jit_cache_destroy(E->jit);
   if (E->events_ok)
     for (int i = 0; i < N_EVENTS; i++) hipEventDestroy(E->ev[i]);
   delete E;
@@ -231,6 +249,9 @@ extern "C" CE_EXPORT int ce_engine_stage_times(ce_engine* E, double* ms, int n_m
   }
   return k;
 }
+
+// 1 when the last evaluation's simulation stage ran the runtime-specialised kernel, else 0.
+extern "C" CE_EXPORT int ce_engine_last_sim_jit(ce_engine* E) { return E ? E->last_sim_jit : 0; }
 
 extern "C" CE_EXPORT int ce_detmath_device(int fn, const double* x, const double* y, double* out, int n, void* stream) {
   ce::launch_detmath(fn, x, y, out, n, (hipStream_t)stream);

@@ -139,6 +139,9 @@ CE_EXPORT int ce_engine_set_diagnostics(ce_engine* eng, float* d_yplant, float* 
 
 /* Per-stage timings of the last ce_evaluate (milliseconds, from HIP events on its stream). */
 CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
+/* 1 when the last ce_evaluate ran the runtime-specialised (hipRTC) simulation kernel.  The engine
+ * specialises the per-step code of a problem unless CE_JIT=0 is set in the environment. */
+CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
 
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */

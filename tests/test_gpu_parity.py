@@ -30,12 +30,16 @@ def assert_rows_equal(got, ref, label):
         pytest.fail(msg)
 
 
+@pytest.mark.parametrize("jit", ["1", "0"])
 @pytest.mark.parametrize("name,n", [("c1", 1024), ("c2", 1024), ("c3", 512)])
-def test_fp64_chain_matches_oracle(engine, name, n):
+def test_fp64_chain_matches_oracle(engine, name, n, jit, monkeypatch):
+    from controllerengine_amd import _abi
+    monkeypatch.setenv("CE_JIT", jit)
     p = workloads.CONFIGS[name](n=n)
-    print("case", name, n, flush=True)
+    print("case", name, n, "jit", jit, flush=True)
     _, ref64, _ = oracle.evaluate(p)
     got32, got64 = engine.evaluate(p, PRECISION_FP64)
+    assert _abi.lib().ce_engine_last_sim_jit(engine.h) == int(jit)
     assert_rows_equal(got64, ref64, name)
     assert np.array_equal(got32.view(np.uint32), ref64.astype(np.float32).view(np.uint32))
 
