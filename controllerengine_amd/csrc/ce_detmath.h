@@ -9,6 +9,7 @@
 //
 // Methods (standard, published):
 //   exp    Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-13 Taylor polynomial, 2^k scaling.
+//   expm1  degree-17 Taylor polynomial on |x| < 1/2, exp(x) - 1 beyond.
 //   log    x = 2^e m, m in [sqrt(1/2), sqrt 2); log m = 2 atanh(s), s = (m-1)/(m+1), odd series in s.
 //   sin/cos  three-part Cody-Waite reduction by pi/2, Taylor polynomials on |r| <= pi/4, quadrant map.
 //   atan2  ratio t in [0,1], nearest breakpoint c = j/8, atan t = atan c + atan((t-c)/(1+tc)),
@@ -126,6 +127,29 @@ CE_HD double log(double x) {
 }
 
 CE_HD double log10(double x) { return log(x) * INV_LN10; }
+
+// exp(x) - 1 without cancellation near 0: Taylor series (degree 17) on |x| < 1/2, exp(x) - 1 beyond.
+CE_HD double expm1(double x) {
+  if (!(fabs_(x) < 0.5)) return exp(x) - 1.0;
+  double p = 0x1.6827863b97d97p-53;         // 1/18! ... evaluated as x + x^2 q(x)
+  p = fma_(p, x, 0x1.952c77030ad4ap-49);    // 1/17!
+  p = fma_(p, x, 0x1.ae7f3e733b81fp-45);    // 1/16!
+  p = fma_(p, x, 0x1.ae7f3e733b81fp-41);    // 1/15!
+  p = fma_(p, x, 0x1.93974a8c07c9dp-37);    // 1/14!
+  p = fma_(p, x, 0x1.6124613a86d09p-33);    // 1/13!
+  p = fma_(p, x, 0x1.1eed8eff8d898p-29);    // 1/12!
+  p = fma_(p, x, 0x1.ae64567f544e4p-26);    // 1/11!
+  p = fma_(p, x, 0x1.27e4fb7789f5cp-22);    // 1/10!
+  p = fma_(p, x, 0x1.71de3a556c734p-19);    // 1/9!
+  p = fma_(p, x, 0x1.a01a01a01a01ap-16);    // 1/8!
+  p = fma_(p, x, 0x1.a01a01a01a01ap-13);    // 1/7!
+  p = fma_(p, x, 0x1.6c16c16c16c17p-10);    // 1/6!
+  p = fma_(p, x, 0x1.1111111111111p-7);     // 1/5!
+  p = fma_(p, x, 0x1.5555555555555p-5);     // 1/4!
+  p = fma_(p, x, 0x1.5555555555555p-3);     // 1/3!
+  p = fma_(p, x, 0.5);                      // 1/2!
+  return fma_(p * x, x, x);
+}
 
 // ---------------------------------------------------------------------------------------------
 // sin and cos of r, |r| <= pi/4 (+ a little): Taylor series, truncation below 1e-19 relative.

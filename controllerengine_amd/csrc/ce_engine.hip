@@ -24,8 +24,9 @@ bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_info_init(const float* in, double* out, size_t count, hipStream_t st);
 void launch_info_final(const double* in, float* out, size_t count, hipStream_t st);
-bool launch_mixed_chain(const ce_config& cfg, const ProblemDev* P, const StageBufs& B, hipStream_t st,
-                        hipEvent_t* ev, std::string& err);
+bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_compact(const StageBufs& B, int all, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
@@ -57,6 +58,7 @@ struct ce_engine {
     size_t cap = 0;
   };
   Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
+  Buf mr32, refined, rlist, rcount, rslots, rcnt;
   float* diag_y = nullptr;
   float* diag_lina = nullptr;
   float* diag_eig = nullptr;
@@ -132,11 +134,22 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   B.open_a = (double*)ensure(E->open_a, nn * ns * ns * sizeof(double));
   if (precision == CE_PRECISION_FP64) B.ybuf = (double*)ensure(E->ybuf, nn * (size_t)H.fit_count * sizeof(double));
   else B.ybuf32 = (float*)ensure(E->ybuf32, nn * (size_t)H.fit_count * sizeof(float));
-  B.mag_raw = (double*)ensure(E->mag_raw, nn * (size_t)F * 2 * sizeof(double));
+  const bool mixed = precision == CE_PRECISION_MIXED;
+  if (precision != CE_PRECISION_FP64 && !mixed)
+    throw std::runtime_error("ce_evaluate: unknown precision " + std::to_string(precision));
+  B.mag_raw = (double*)ensure(E->mag_raw, nn * (size_t)F * (mixed ? 3 : 2) * sizeof(double));
   B.bmin = (unsigned long long*)ensure(E->bmin, nn * sizeof(unsigned long long));
   B.sweep_fail = (int32_t*)ensure(E->sfail, nn * sizeof(int32_t));
   B.list = (int32_t*)ensure(E->list, nn * sizeof(int32_t));
   B.count = (int32_t*)ensure(E->count, sizeof(int32_t));
+  if (mixed) {
+    B.mr32 = (float*)ensure(E->mr32, nn * (size_t)F * 3 * sizeof(float));
+    B.refined = (int8_t*)ensure(E->refined, ((nn * (size_t)F + 3) / 4) * 4);
+    B.rlist = (int2*)ensure(E->rlist, nn * RMAX * sizeof(int2));
+    B.rcount = (int32_t*)ensure(E->rcount, sizeof(int32_t));
+    B.rslots = (int32_t*)ensure(E->rslots, nn * RMAX * sizeof(int32_t));
+    B.rcnt = (int32_t*)ensure(E->rcnt, nn * sizeof(int32_t));
+  }
   B.diag = (E->diag_y || E->diag_lina || E->diag_eig || E->diag_ata || E->diag_atb) ? 1 : 0;
   B.d_y = E->diag_y; B.d_lina = E->diag_lina; B.d_eig = E->diag_eig; B.d_ata = E->diag_ata; B.d_atb = E->diag_atb;
   if (B.diag) {
@@ -146,38 +159,47 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
   launch_fill_i32(B.sweep_fail, 0, nn, st);
   launch_fill_i32(B.count, 0, 1, st);
+  if (mixed) {
+    launch_fill_i32((int32_t*)B.refined, 0, (nn * (size_t)F + 3) / 4, st);
+    launch_fill_i32(B.rcount, 0, 1, st);
+    launch_fill_i32(B.rcnt, 0, nn, st);
+  }
   launch_info_init(d_info_in, B.info, nn * H.info_w, st);
   E->last_stream = st;
-  if (precision == CE_PRECISION_FP64) {
-    HIPCHK(hipEventRecord(E->ev[0], st));
-    launch_eig_fp64(P, B, st);
-    launch_compact(B, B.d_y != nullptr, st);
-    HIPCHK(hipEventRecord(E->ev[1], st));
-    void* fn = nullptr;
-    E->last_sim_jit = 0;
-    if (jit_enabled()) {
-      if (!E->jit) E->jit = jit_cache_create();
-      std::string jerr;
-      if (jit_get(E->jit, gen_sim_source(H, false, 0), &fn, jerr)) {
-        if (!jit_launch_sim(fn, (n + 63) / 64, st, B.gains, B.flags, B.list, B.count, B.ybuf, B.info, B.d_y, B.mode_re,
-                            B.mode_im, B.uniq, nullptr, n))
-          throw std::runtime_error("launch of the specialised simulation kernel failed");
-        E->last_sim_jit = 1;
-      } else {
-        fn = nullptr;
-      }
+
+  // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
+  HIPCHK(hipEventRecord(E->ev[0], st));
+  launch_eig_fp64(P, B, st);
+  launch_compact(B, B.d_y != nullptr, st);
+  HIPCHK(hipEventRecord(E->ev[1], st));
+  // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
+  void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;
+  void* fn = nullptr;
+  E->last_sim_jit = 0;
+  if (jit_enabled()) {
+    if (!E->jit) E->jit = jit_cache_create();
+    std::string jerr;
+    if (jit_get(E->jit, gen_sim_source(H, mixed), &fn, jerr)) {
+      if (!jit_launch_sim(fn, (n + 63) / 64, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n))
+        throw std::runtime_error("launch of the specialised simulation kernel failed");
+      E->last_sim_jit = 1;
+    } else {
+      fn = nullptr;
     }
-    if (!fn && !launch_sim_fp64(ns, H.nd, P, B, st)) throw std::runtime_error("no simulation kernel for these dimensions");
-    HIPCHK(hipEventRecord(E->ev[2], st));
-    if (!launch_fit_fp64(nt, P, B, st)) throw std::runtime_error("no fit kernel for these dimensions");
-    HIPCHK(hipEventRecord(E->ev[3], st));
-    if (!launch_sweep_fp64(ns, F, P, B, st)) throw std::runtime_error("no sweep kernel for these dimensions");
-    HIPCHK(hipEventRecord(E->ev[4], st));
-  } else if (precision == CE_PRECISION_MIXED) {
-    if (!launch_mixed_chain(cfg, P, B, st, E->ev, err)) throw std::runtime_error(err);
-  } else {
-    throw std::runtime_error("ce_evaluate: unknown precision " + std::to_string(precision));
   }
+  if (!fn) {
+    const bool ok = mixed ? launch_sim_mixed(ns, H.nd, P, B, st) : launch_sim_fp64(ns, H.nd, P, B, st);
+    if (!ok) throw std::runtime_error("no simulation kernel for these dimensions");
+  }
+  HIPCHK(hipEventRecord(E->ev[2], st));
+  // stage 3: modal fit and behaviour score
+  if (!(mixed ? launch_fit_mixed(nt, P, B, st) : launch_fit_fp64(nt, P, B, st)))
+    throw std::runtime_error("no fit kernel for these dimensions");
+  HIPCHK(hipEventRecord(E->ev[3], st));
+  // stage 4: margin sweep, crossings and total score
+  if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st)))
+    throw std::runtime_error("no sweep kernel for these dimensions");
+  HIPCHK(hipEventRecord(E->ev[4], st));
   HIPCHK(hipGetLastError());
   launch_info_final(B.info, d_info_out, nn * H.info_w, st);
   if (d_info64_out)
@@ -208,7 +230,8 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (!E) return;
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
-                  &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count})
+                  &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
+                  &E->rcount, &E->rslots, &E->rcnt})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

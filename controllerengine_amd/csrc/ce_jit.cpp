@@ -10,9 +10,7 @@
 //
 // Two variants are generated:
 //   fp64   double state; stores the fit window of y (SoA) and yss/overshoot (the FP64 chain).
-//   mixed  float state; accumulates, in double, sum_t phi(t) y(t) and sum_t phi(t) over the fit
-//          window with the damped-sinusoid basis advanced by a complex recurrence, so that no
-//          trajectory is stored (the mixed chain).
+//   mixed  float state; stores the fit window of y as float (the mixed chain).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -97,7 +95,7 @@ struct Gen {
 
 }  // namespace
 
-std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes) {
+std::string gen_sim_source(const ProblemDev& P, bool mixed) {
   Gen g;
   g.f32 = mixed;
   const int ns = P.ns, nd = P.nd, nw = P.nw;
@@ -105,9 +103,8 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes) {
   const std::string T = g.T();
   o << "// generated: closed-loop simulation, ns=" << ns << " nd=" << nd << " nw=" << nw << (mixed ? " float" : " double") << "\n";
   o << "extern \"C\" __global__ __launch_bounds__(64) void ce_sim(const float* __restrict__ gains, const int* __restrict__ flags,\n"
-       "    const int* __restrict__ list, const int* __restrict__ count, double* __restrict__ ybuf, double* __restrict__ info,\n"
-       "    float* __restrict__ dy, const double* __restrict__ mode_re, const double* __restrict__ mode_im,\n"
-       "    const int* __restrict__ uniq, double* __restrict__ acc, int n) {\n";
+       "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
+       "    float* __restrict__ dy, int n) {\n";
   o << "  const int idx = blockIdx.x * 64 + threadIdx.x;\n  if (idx >= *count) return;\n  const int c = list[idx];\n";
   o << "  const bool unstable = (flags[c] & 1) != 0;\n";
   o << "  const float* g = gains + (long)c * " << P.cuda_w << ";\n";
@@ -118,26 +115,7 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes) {
   o << "  " << T << " ysum = " << g.zero() << ", ymax = -" << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ", ymin = "
     << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ";\n";
   const int f0 = P.fit_start_id, f1 = P.fit_start_id + P.fit_count, s0 = P.steady_start_id;
-  if (!mixed) {
-    o << "  double* yb = ybuf + idx;\n";
-  } else {
-    // damped-sinusoid basis phi_2e + i phi_2e+1 = exp((re+i im) tval - shift), advanced by a complex
-    // rotation each sample; accumulators sum phi*y and sum phi (double)
-    o << "  const int u = unstable ? 0 : uniq[c];\n";
-    for (int e = 0; e < modes; e++) {
-      o << "  double br" << e << " = 0.0, bi" << e << " = 0.0, sr" << e << " = 0.0, si" << e << " = 0.0;\n";
-      o << "  double ayr" << e << " = 0.0, ayi" << e << " = 0.0, asr" << e << " = 0.0, asi" << e << " = 0.0;\n";
-      o << "  if (" << e << " < u) {\n";
-      o << "    const double re = mode_re[(long)" << e << " * n + c], im = mode_im[(long)" << e << " * n + c];\n";
-      o << "    const double sh = __builtin_fmax(re * " << lit(P.fit_t0) << ", re * " << lit(P.fit_t1) << ");\n";
-      o << "    const double t0 = (double)" << f0 << " * " << lit(P.ts) << ";\n";
-      o << "    const double m0 = exp(re * t0 - sh);\n";
-      o << "    br" << e << " = m0 * cos(im * t0); bi" << e << " = m0 * sin(im * t0);\n";
-      o << "    const double ms = exp(re * " << lit(P.ts) << ");\n";
-      o << "    sr" << e << " = ms * cos(im * " << lit(P.ts) << "); si" << e << " = ms * sin(im * " << lit(P.ts) << ");\n";
-      o << "  }\n";
-    }
-  }
+  o << "  " << T << "* yb = (" << T << "*)ybuf + idx;\n";
   o << "  for (int t = 0; t < " << P.T << "; t++) {\n";
   std::vector<std::string> sxp, sxh, snl;
   for (int i = 0; i < nd; i++) sxp.push_back("xp" + std::to_string(i));
@@ -145,18 +123,7 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes) {
   const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "");
   o << "    const " << T << " y = " << y << ";\n";
   o << "    if (dy) dy[(long)c * " << P.T << " + t] = (float)y;\n";
-  if (!mixed) {
-    o << "    if (t >= " << f0 << " && t < " << f1 << ") yb[(long)(t - " << f0 << ") * n] = y;\n";
-  } else {
-    o << "    if (t >= " << f0 << " && t < " << f1 << ") {\n      const double yd = (double)y;\n";
-    for (int e = 0; e < modes; e++) {
-      o << "      ayr" << e << " = __builtin_fma(br" << e << ", yd, ayr" << e << "); ayi" << e << " = __builtin_fma(bi" << e << ", yd, ayi" << e << ");\n";
-      o << "      asr" << e << " = asr" << e << " + br" << e << "; asi" << e << " = asi" << e << " + bi" << e << ";\n";
-      o << "      { const double nr = br" << e << " * sr" << e << " - bi" << e << " * si" << e << "; bi" << e << " = br" << e << " * si" << e
-        << " + bi" << e << " * sr" << e << "; br" << e << " = nr; }\n";
-    }
-    o << "    }\n";
-  }
+  o << "    if (t >= " << f0 << " && t < " << f1 << ") yb[(long)(t - " << f0 << ") * n] = y;\n";
   o << "    if (t >= " << s0 << ") ysum = ysum + y;\n";
   o << "    ymax = " << g.fmax() << "(ymax, y); ymin = " << g.fmin() << "(ymin, y);\n";
   o << "    " << T << " kx = " << g.zero() << ";\n";
@@ -193,11 +160,6 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes) {
     o << "  fr[2] = pk / __builtin_fmax(yss * yss, " << lit(EPS6) << ") - 1.0;\n  fr[3] = yss;\n";
   } else {
     o << "  fr[2] = (double)(pk / __builtin_fmaxf(yss * yss, " << litf((float)EPS6) << ") - 1.0f);\n  fr[3] = (double)yss;\n";
-    // acc layout per candidate slot idx: [4*modes] = (sum phi_c y, sum phi_s y, sum phi_c, sum phi_s) x mode, SoA by n
-    for (int e = 0; e < modes; e++) {
-      o << "  acc[(long)(" << 4 * e << ") * n + idx] = ayr" << e << "; acc[(long)(" << 4 * e + 1 << ") * n + idx] = ayi" << e << ";\n";
-      o << "  acc[(long)(" << 4 * e + 2 << ") * n + idx] = asr" << e << "; acc[(long)(" << 4 * e + 3 << ") * n + idx] = asi" << e << ";\n";
-    }
   }
   o << "}\n";
   return o.str();
@@ -273,9 +235,8 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
 }
 
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, double* ybuf, double* info, float* dy, const double* mode_re,
-                    const double* mode_im, const int* uniq, double* acc, int n) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &mode_re, &mode_im, &uniq, &acc, &n};
+                    const int* count, void* ybuf, double* info, float* dy, int n) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

@@ -13,12 +13,10 @@ struct JitCache;
 JitCache* jit_cache_create();
 void jit_cache_destroy(JitCache* c);
 bool jit_enabled();
-// HIP source of the simulation kernel `ce_sim` for problem P.  mixed: float state plus the fused
-// basis/right-hand-side accumulation for `modes` modes.
-std::string gen_sim_source(const ProblemDev& P, bool mixed, int modes);
+// HIP source of the simulation kernel `ce_sim` for problem P (mixed: float state and trajectory).
+std::string gen_sim_source(const ProblemDev& P, bool mixed);
 bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err);
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, double* ybuf, double* info, float* dy, const double* mode_re,
-                    const double* mode_im, const int* uniq, double* acc, int n);
+                    const int* count, void* ybuf, double* info, float* dy, int n);
 
 }  // namespace ce

@@ -14,15 +14,13 @@
 //        -> k_scan_fp64       one candidate per lane: phase unwrap, crossings, score (sequential in w).
 #include <hip/hip_runtime.h>
 
+#include "ce_fitsolve_dev.h"
 #include "ce_linalg_dev.h"
+#include "ce_sim_dev.h"
+#include "ce_sweep_dev.h"
 #include "ce_stage.h"
 
 namespace ce {
-
-template <typename Tp>
-__device__ __forceinline__ const Tp* tab(const ProblemDev* P, int64_t off) {
-  return reinterpret_cast<const Tp*>(reinterpret_cast<const uint8_t*>(P) + off);
-}
 
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_info_init(const float* __restrict__ in, double* __restrict__ out, size_t count) {
@@ -144,128 +142,6 @@ __global__ __launch_bounds__(64) void k_eig_fp64(const ProblemDev* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Instruction chain (reference src/main.cu:260-291).  ops/count/col are wave-uniform.
-template <int DIM, int MAXCOUNT, bool HAS_SEED>
-__device__ __forceinline__ double interp_dev(const Ins* __restrict__ ops, int count, const double (&src)[DIM], double seed) {
-  double result = 0.0;
-  bool started = false;
-#pragma unroll
-  for (int r = 0; r < MAXCOUNT; r++) {
-    if (r >= count) break;
-    const Ins in = ops[r];
-    if (in.op == 0) continue;
-    double pos = src[0];
-#pragma unroll
-    for (int k = 1; k < DIM; k++)
-      if (in.col == k) pos = src[k];
-    double prev;
-    if (started) prev = result;
-    else if (HAS_SEED) prev = (seed == seed) ? seed : pos;
-    else prev = pos;
-    const double a = in.akind == K_CONST ? in.aval : (in.akind == K_STATE ? pos : prev);
-    const double b = in.bkind == K_CONST ? in.bval : (in.bkind == K_STATE ? pos : prev);
-    switch (in.op) {
-      case 1: result = a + b; break;
-      case 2: result = a - b; break;
-      case 3: result = a * b; break;
-      case 4: result = a / b; break;
-      case 5: result = cedm::fmin_(cedm::fmax_(prev, a), b); break;
-      case 6: result = (double)((0.0 < a) - (a < 0.0)); break;
-      default: break;
-    }
-    started = true;
-  }
-  return result;
-}
-
-// Closed-loop simulation, reference src/discrete_solves.cu:12-79, in double without hi/lo pairs.
-template <int NS, int ND>
-__global__ __launch_bounds__(64) void k_sim_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int idx = blockIdx.x * 64 + threadIdx.x;
-  if (idx >= *B.count) return;
-  const int c = B.list[idx];
-  const bool unstable = (B.flags[c] & F_UNSTABLE) != 0;
-  const int n = B.n;
-  const float* g = B.gains + (size_t)c * P->cuda_w;
-  double K[NS], L[NS];
-#pragma unroll
-  for (int i = 0; i < NS; i++) { K[i] = g[i]; L[i] = g[NS + i]; }
-  const double N = g[2 * NS];
-  double xh[NS], xp[ND];
-#pragma unroll
-  for (int i = 0; i < NS; i++) xh[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < ND; i++) xp[i] = 0.0;
-  const int T = P->T, f0 = P->fit_start_id, f1 = P->fit_start_id + P->fit_count;
-  const int s0 = P->steady_start_id, k0 = P->ref_step_id, nw = P->nw;
-  const int dc_n = P->dc_n, u_n = P->u_n;
-  const double amp = P->ref_amp, sd = P->sd;
-  double ysum = 0.0, ymax = -cedm::inf(), ymin = cedm::inf();
-  double* ybuf = B.ybuf + idx;
-  float* dy = (B.diag && B.d_y) ? B.d_y + (size_t)c * T : nullptr;
-  for (int t = 0; t < T; t++) {
-    const double y = interp_dev<ND, CE_MAX_DC_INSTRUCTIONS, false>(P->dc_ops, dc_n, xp, 0.0);
-    if (dy) dy[t] = (float)y;
-    if (t >= f0 && t < f1) ybuf[(size_t)(t - f0) * n] = y;
-    if (t >= s0) ysum = ysum + y;
-    ymax = cedm::fmax_(ymax, y);
-    ymin = cedm::fmin_(ymin, y);
-    double kx = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; k++) kx = cedm::fma_(K[k], xh[k], kx);
-    const double r = (t < k0) ? 0.0 : amp;
-    const double u_raw = cedm::fma_(N, r, -kx);
-    const double u = interp_dev<NS, CE_MAX_U_INSTRUCTIONS, true>(P->u_ops, u_n, xh, u_raw);
-    double cx = 0.0;
-#pragma unroll
-    for (int k = 0; k < NS; k++) cx = cedm::fma_(P->sc[k], xh[k], cx);
-    const double yerr = y - cedm::fma_(sd, u, cx);
-    double dh[NS];
-#pragma unroll
-    for (int i = 0; i < NS; i++) {
-      double acc = cedm::fma_(L[i], yerr, P->sb[i] * u);
-#pragma unroll
-      for (int k = 0; k < NS; k++) acc = cedm::fma_(P->sai[i * NS + k], xh[k], acc);
-      dh[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < NS; i++) xh[i] = xh[i] + dh[i];
-    double nls[ND + 1];
-#pragma unroll
-    for (int i = 0; i < ND; i++) nls[i] = xp[i];
-    nls[ND] = y;
-    double W[MAXW];
-#pragma unroll
-    for (int q = 0; q < MAXW; q++) {
-      W[q] = 0.0;
-      if (q < nw) W[q] = interp_dev<ND + 1, CE_MAX_NL_INSTRUCTIONS, false>(P->nl_ops[q], P->nl_n[q], nls, 0.0);
-    }
-    double ph[ND];
-#pragma unroll
-    for (int i = 0; i < ND; i++) {
-      double base = P->dbu[i] * u;
-#pragma unroll
-      for (int q = 0; q < MAXW; q++)
-        if (q < nw) base = cedm::fma_(P->dbw[i * nw + q], W[q], base);
-      double acc = base;
-#pragma unroll
-      for (int k = 0; k < ND; k++) acc = cedm::fma_(P->dai[i * ND + k], xp[k], acc);
-      ph[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < ND; i++) xp[i] = xp[i] + ph[i];
-  }
-  if (unstable) return;
-  // steady-state mean and overshoot (src/modalsetup.cu:52-62).  max_t fl(y_t * yss) equals
-  // fl(max_t y_t * yss) for yss > 0 and fl(min_t y_t * yss) for yss < 0 (rounding is monotonic).
-  const double yss = ysum / (double)(T - s0);
-  const double pk = cedm::fmax_(0.0, ((yss < 0.0) ? ymin : ymax) * yss);
-  double* fr = B.info + (size_t)c * P->info_w + P->info_freqresp;
-  fr[2] = pk / cedm::fmax_(yss * yss, EPS6) - 1.0;
-  fr[3] = yss;
-}
-
-// ------------------------------------------------------------------------------------------------
 // Modal fit: Gram accumulation (src/modalsetup.cu:64-96) + LDLT solve and behaviour score
 // (src/modalsolve.cu:150-203).  NB = number of 8-column blocks of the 2nt-wide normal equations.
 //
@@ -275,22 +151,23 @@ __global__ __launch_bounds__(64) void k_sim_fp64(const ProblemDev* __restrict__ 
 // accumulated by exactly one lane, in time order, as fma(phi_i, phi_j, acc).
 template <int NB>
 struct FitShape {
-  static constexpr int G = (NB == 1) ? 1 : ((NB == 2) ? 4 : 16);
+  static constexpr int G = 16;        // lanes per candidate
   static constexpr int CPW = 64 / G;  // candidates per workgroup
   static constexpr int NCP = 8 * NB;  // padded basis width
+  static constexpr int NM = NCP / 2;  // padded mode count
   static constexpr int NTRI = NCP * (NCP + 1) / 2;
   static constexpr int TB = 8;        // samples staged per LDS round
+  static constexpr int UNITS = (NB == 1) ? 1 : NB + NB * (NB - 1);
 };
-
-__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 
 template <int NB>
 __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
   using S = FitShape<NB>;
-  constexpr int G = S::G, CPW = S::CPW, NCP = S::NCP, TB = S::TB;
+  constexpr int G = S::G, CPW = S::CPW, NCP = S::NCP, NM = S::NM, TB = S::TB;
   __shared__ double phi[CPW][TB][NCP];
   __shared__ double gram[CPW][S::NTRI];
   __shared__ double vec[CPW][5][NCP];  // atb, diag, subdiag, col scale, solution
+  __shared__ double modes[CPW][S::NM][3];
 
   const int lane = threadIdx.x;
   const int slot = lane / G, g = lane % G;
@@ -308,35 +185,38 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
     active = stable && (row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)));
     u = B.uniq[c];
   }
-  // unit owned by this lane
+  // tile owned by this lane: lane g < NB owns diagonal block g (its lower triangle as an 8x4
+  // rectangle plus a 4x4 square) and the right-hand side of those rows; lanes NB.. own the two 8x4
+  // halves of one off-diagonal block each; the rest accumulate nothing
   int rb = 0, cb = 0, coff = 0;
-  bool diag_unit = false, idle = false;
+  bool diag_unit = false, idle = g >= S::UNITS;
   if (g < NB) {
     diag_unit = true;
     rb = g;
     cb = g;
-  } else {
+  } else if (!idle) {
     int p = (g - NB) >> 1;
     coff = ((g - NB) & 1) * 4;
     rb = 1;
     while (p >= rb) { p -= rb; rb++; }
     cb = p;
-    idle = rb >= NB;
   }
-  // this candidate's modes (u <= nt), read by the lanes that evaluate them
+  // basis evaluations of a TB-sample block: NM modes x TB samples spread over the G lanes; the
+  // candidate's mode table (decay rate, frequency, shift) is staged in LDS
+  constexpr int EVALS = NM * TB / G;
   const double ts = P->ts, t0 = P->fit_t0, t1 = P->fit_t1;
-  double mre[(MAXT + G - 1) / G], mim[(MAXT + G - 1) / G], msh[(MAXT + G - 1) / G];
-  constexpr int MPL = (MAXT + G - 1) / G;  // modes per lane
-#pragma unroll
-  for (int q = 0; q < MPL; q++) {
-    const int e = g + q * G;
-    mre[q] = 0.0; mim[q] = 0.0; msh[q] = 0.0;
+  for (int e = g; e < NM; e += G) {
+    double re = 0.0, im = 0.0, sh = 0.0;
     if (active && e < u) {
-      mre[q] = B.mode_re[(size_t)e * n + c];
-      mim[q] = B.mode_im[(size_t)e * n + c];
-      msh[q] = cedm::fmax_(mre[q] * t0, mre[q] * t1);
+      re = B.mode_re[(size_t)e * n + c];
+      im = B.mode_im[(size_t)e * n + c];
+      sh = cedm::fmax_(re * t0, re * t1);
     }
+    modes[slot][e][0] = re;
+    modes[slot][e][1] = im;
+    modes[slot][e][2] = sh;
   }
+  __syncthreads();
   double R1[8][4], R2[4][4], AB[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
@@ -353,28 +233,24 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   const int f0 = P->fit_start_id, cnt = P->fit_count;
   const double* ycol = B.ybuf + (active ? idx : 0);
   const int rbase = rb * 8, cbase = cb * 8 + coff;
+  const int nc = 2 * u;
   for (int tb = 0; tb < cnt; tb += TB) {
-    // stage basis values of TB samples
-#pragma unroll
-    for (int s = 0; s < TB; s++) {
+#pragma unroll 1
+    for (int q = 0; q < EVALS; q++) {
+      const int k = g + q * G;
+      const int e = k % NM, s = k / NM;
       const int t = tb + s;
-      const double tval = (double)(f0 + t) * ts;
-#pragma unroll
-      for (int q = 0; q < MPL; q++) {
-        const int e = g + q * G;
-        if (e < NCP / 2) {
-          double cv = 0.0, sv = 0.0;
-          if (active && e < u && t < cnt) {
-            const double decay = cedm::exp(cedm::fma_(mre[q], tval, -msh[q]));
-            double sn, cs;
-            cedm::sincos(mim[q] * tval, sn, cs);
-            cv = decay * cs;
-            sv = decay * sn;
-          }
-          phi[slot][s][2 * e] = cv;
-          phi[slot][s][2 * e + 1] = sv;
-        }
+      double cv = 0.0, sv = 0.0;
+      if (active && e < u && t < cnt) {
+        const double tval = (double)(f0 + t) * ts;
+        const double decay = cedm::exp(cedm::fma_(modes[slot][e][0], tval, -modes[slot][e][2]));
+        double sn, cs;
+        cedm::sincos(modes[slot][e][1] * tval, sn, cs);
+        cv = decay * cs;
+        sv = decay * sn;
       }
+      phi[slot][s][2 * e] = cv;
+      phi[slot][s][2 * e + 1] = sv;
     }
     __syncthreads();
     if (active && !idle) {
@@ -400,7 +276,8 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
             for (int j = 0; j < 4; j++) R2[i][j] = cedm::fma_(pr[4 + i], pc2[j], R2[i][j]);
           const double yv = ycol[(size_t)t * n] - yss;
 #pragma unroll
-          for (int i = 0; i < 8; i++) AB[i] = cedm::fma_(pr[i], yv, AB[i]);
+          for (int i = 0; i < 8; i++)
+            if (rbase + i < nc) AB[i] = cedm::fma_(pr[i], yv, AB[i]);
         }
       }
     }
@@ -449,128 +326,10 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   }
   double* sc = row + P->info_scoring;
   if (sc[0] != 0.0) return;  // modalsolve skips flagged candidates (src/modalsolve.cu:112)
-
-  for (int i = 0; i < nc2; i++) {
-    dg[i] = G2[tri(i, i)];
-    sd[i] = (i + 1 < nc2) ? G2[tri(i + 1, i)] : 0.0;
-    const double s = cedm::sqrt_(dg[i]);
-    cs[i] = (s > EPS6) ? s : 1.0;
-  }
-  // scaled system D^-1 G D^-1 (lower triangle) and right-hand side D^-1 b
-  for (int i = 0; i < nc2; i++) {
-    const double inv_i = 1.0 / cs[i];
-    for (int j = 0; j <= i; j++) {
-      const double inv_j = 1.0 / cs[j];
-      G2[tri(i, j)] = (inv_i * G2[tri(i, j)]) * inv_j;
-    }
-    x[i] = atb[i] / cs[i];
-  }
-  // LDL^T with symmetric pivoting on the largest remaining diagonal (lower storage)
-  int trans[MAXNC];
-  bool all_zero = false;
-  const int N2 = nc2;
-  for (int k = 0; k < N2; k++) {
-    int idx = k;
-    double best = cedm::fabs_(G2[tri(k, k)]);
-    for (int i = k + 1; i < N2; i++) {
-      const double a = cedm::fabs_(G2[tri(i, i)]);
-      if (a > best) { best = a; idx = i; }
-    }
-    trans[k] = idx;
-    if (k != idx) {
-      for (int j = 0; j < k; j++) { const double t = G2[tri(k, j)]; G2[tri(k, j)] = G2[tri(idx, j)]; G2[tri(idx, j)] = t; }
-      for (int i = idx + 1; i < N2; i++) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(i, idx)]; G2[tri(i, idx)] = t; }
-      { const double t = G2[tri(k, k)]; G2[tri(k, k)] = G2[tri(idx, idx)]; G2[tri(idx, idx)] = t; }
-      for (int i = k + 1; i < idx; i++) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(idx, i)]; G2[tri(idx, i)] = t; }
-    }
-    if (k > 0) {
-      double tmp[MAXNC];
-      for (int j = 0; j < k; j++) tmp[j] = G2[tri(j, j)] * G2[tri(k, j)];
-      double acc = 0.0;
-      for (int j = 0; j < k; j++) acc = cedm::fma_(G2[tri(k, j)], tmp[j], acc);
-      G2[tri(k, k)] = G2[tri(k, k)] - acc;
-      for (int i = k + 1; i < N2; i++) {
-        double a = 0.0;
-        for (int j = 0; j < k; j++) a = cedm::fma_(G2[tri(i, j)], tmp[j], a);
-        G2[tri(i, k)] = G2[tri(i, k)] - a;
-      }
-    }
-    const double akk = G2[tri(k, k)];
-    const bool valid = cedm::fabs_(akk) > 0.0;
-    if (k == 0 && !valid) { all_zero = true; break; }
-    if (valid)
-      for (int i = k + 1; i < N2; i++) G2[tri(i, k)] = G2[tri(i, k)] / akk;
-  }
-  if (all_zero) {
-    for (int i = 0; i < N2; i++) x[i] = 0.0;
-  } else {
-    for (int k = 0; k < N2; k++) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
-    for (int i = 0; i < N2; i++) {
-      double acc = 0.0;
-      for (int j = 0; j < i; j++) acc = cedm::fma_(G2[tri(i, j)], x[j], acc);
-      x[i] = x[i] - acc;
-    }
-    for (int i = 0; i < N2; i++) {
-      const double d = G2[tri(i, i)];
-      x[i] = (cedm::fabs_(d) > DBL_MIN) ? x[i] / d : 0.0;
-    }
-    for (int i = N2 - 1; i >= 0; i--) {
-      double acc = 0.0;
-      for (int j = i + 1; j < N2; j++) acc = cedm::fma_(G2[tri(j, i)], x[j], acc);
-      x[i] = x[i] - acc;
-    }
-    for (int k = N2 - 1; k >= 0; k--) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
-  }
-  for (int i = 0; i < N2; i++) x[i] = x[i] / cs[i];
-
-  // per-mode energy, energy-weighted wn and zeta of the oscillatory modes (src/modalsolve.cu:168-190)
-  double* md = row + P->info_modal;
-  double* fr = row + P->info_freqresp;
-  double esum = 0.0, lw = 0.0, zw = 0.0;
-  for (int e = 0; e < u; e++) {
-    const int id = 2 * e;
-    const double c0 = x[id], c1 = x[id + 1];
-    const double g00 = dg[id], g01 = sd[id], g11 = dg[id + 1];
-    const double energy = ((c0 * c0) * g00 + ((2.0 * c0) * c1) * g01) + (c1 * c1) * g11;
-    const double re_e = B.mode_re[(size_t)e * n + c], im_e = B.mode_im[(size_t)e * n + c];
-    const double wn_e = cedm::fmax_(cedm::sqrt_(cedm::fma_(re_e, re_e, im_e * im_e)), EPS6);
-    md[id] = wn_e;
-    md[id + 1] = -re_e / wn_e;
-    md[2 * nt + e] = energy;
-    if (im_e < EPS6) continue;
-    esum = esum + energy;
-    lw = lw + energy * cedm::log(wn_e);
-    zw = zw + energy * (-re_e / wn_e);
-  }
-  md[3 * nt] = (double)u;
-  double wn, zeta;
-  if (esum <= EPS6) { wn = EPS6; zeta = 0.0; }
-  else { wn = cedm::exp(lw / esum); zeta = zw / esum; }
-  const double dmp = P->damping, d2 = dmp * dmp;
-  const double inner = ((4.0 * d2) * d2 - 4.0 * d2) + 2.0;
-  const double twn = ((2.0 * PI_D) * P->bandwidth) / cedm::sqrt_((1.0 - 2.0 * d2) + cedm::sqrt_(inner));
-  const double lr = cedm::log(wn / twn);
-  double beh = lr * lr + ((zeta - dmp) * (zeta - dmp) / dmp) / dmp;
-  const double os = P->overshoot, sse = P->sse;
-  const double err_os = cedm::fmax_(0.0, fr[2] - os) / os;
-  beh = beh + err_os * err_os;
-  const double err_ss = cedm::fmax_(0.0, cedm::fabs_(fr[3] / P->ref_amp - 1.0) - sse) / sse;
-  beh = beh + err_ss * err_ss;
-  sc[0] = cedm::isfinite_(beh) ? beh : P->eig_fail_err;
-  sc[1] = sc[0];
-  fr[0] = wn;
-  fr[1] = zeta;
+  solve_and_score(P, B, c, u, row, G2, atb, dg, sd, cs, x);
 }
 
 // ------------------------------------------------------------------------------------------------
-// Margin sweep, one frequency per lane (src/allmargin.cu:100-117): G(z) = K^T (zI - A1)^-1 L by
-// elimination with partial pivoting in registers; stores |P G| and arg(P G); min b(w) by atomics.
-__device__ __forceinline__ bool margin_skip(const ProblemDev* P, const StageBufs& B, int c) {
-  const int f = B.flags[c];
-  if (f & (F_NOMINAL_SKIPPED | F_NOMINAL_UNSTABLE)) return true;
-  return B.info[(size_t)c * P->info_w + P->info_scoring] >= P->unstable_err;
-}
-
 template <int NS>
 __global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
   const int c = blockIdx.x;
@@ -579,103 +338,11 @@ __global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict_
   const int F = P->F;
   if (w >= F) return;
   if (!tab<int32_t>(P, P->off_pok)[w]) { atomicOr(&B.sweep_fail[c], 1); return; }
-  const double* oa = B.open_a + (size_t)c * NS * NS;
-  const float* gg = B.gains + (size_t)c * P->cuda_w;
-  const double zr = tab<double>(P, P->off_zr)[w], zi = tab<double>(P, P->off_zi)[w];
-  double Mr[NS][NS], Mi[NS][NS], vr[NS], vi[NS];
-#pragma unroll
-  for (int i = 0; i < NS; i++) {
-#pragma unroll
-    for (int j = 0; j < NS; j++) {
-      Mr[i][j] = (i == j ? zr : 0.0) - oa[i * NS + j];
-      Mi[i][j] = (i == j ? zi : 0.0);
-    }
-    vr[i] = (double)gg[NS + i];
-    vi[i] = 0.0;
+  double m, raw, b;
+  if (!sweep_point<NS, double>(P, B.open_a + (size_t)c * NS * NS, B.gains + (size_t)c * P->cuda_w, w, m, raw, b)) {
+    atomicOr(&B.sweep_fail[c], 1);
+    return;
   }
-#pragma unroll
-  for (int k = 0; k < NS; k++) {
-    int p = k;
-    double best = Mr[k][k] * Mr[k][k] + Mi[k][k] * Mi[k][k];
-#pragma unroll
-    for (int i = k + 1; i < NS; i++) {
-      const double s = Mr[i][k] * Mr[i][k] + Mi[i][k] * Mi[i][k];
-      if (s > best) { best = s; p = i; }
-    }
-    if (best != 0.0) {
-      // row swap k <-> p on columns k.. and the right-hand side, by selects
-#pragma unroll
-      for (int i = k + 1; i < NS; i++) {
-        const bool sw = (p == i);
-#pragma unroll
-        for (int j = k; j < NS; j++) {
-          const double ar = Mr[k][j], ai = Mi[k][j], br = Mr[i][j], bi = Mi[i][j];
-          Mr[k][j] = sw ? br : ar; Mi[k][j] = sw ? bi : ai;
-          Mr[i][j] = sw ? ar : br; Mi[i][j] = sw ? ai : bi;
-        }
-        const double ar = vr[k], ai = vi[k], br = vr[i], bi = vi[i];
-        vr[k] = sw ? br : ar; vi[k] = sw ? bi : ai;
-        vr[i] = sw ? ar : br; vi[i] = sw ? ai : bi;
-      }
-      const double pr = Mr[k][k], pim = Mi[k][k];
-      const double dd = pr * pr + pim * pim;
-#pragma unroll
-      for (int i = k + 1; i < NS; i++) {
-        const double ar = Mr[i][k], ai = Mi[i][k];
-        Mr[i][k] = (ar * pr + ai * pim) / dd;
-        Mi[i][k] = (ai * pr - ar * pim) / dd;
-      }
-    }
-#pragma unroll
-    for (int i = k + 1; i < NS; i++) {
-      const double lr = Mr[i][k], li = Mi[i][k];
-#pragma unroll
-      for (int j = k + 1; j < NS; j++) {
-        const double qr = lr * Mr[k][j] - li * Mi[k][j];
-        const double qi = lr * Mi[k][j] + li * Mr[k][j];
-        Mr[i][j] = Mr[i][j] - qr;
-        Mi[i][j] = Mi[i][j] - qi;
-      }
-      const double qr = lr * vr[k] - li * vi[k];
-      const double qi = lr * vi[k] + li * vr[k];
-      vr[i] = vr[i] - qr;
-      vi[i] = vi[i] - qi;
-    }
-  }
-  double xr[NS], xi[NS];
-  bool fin = true;
-#pragma unroll
-  for (int i = NS - 1; i >= 0; i--) {
-    const double pr = Mr[i][i], pim = Mi[i][i];
-    const double dd = pr * pr + pim * pim;
-    xr[i] = (vr[i] * pr + vi[i] * pim) / dd;
-    xi[i] = (vi[i] * pr - vr[i] * pim) / dd;
-#pragma unroll
-    for (int j = 0; j < i; j++) {
-      const double qr = Mr[j][i] * xr[i] - Mi[j][i] * xi[i];
-      const double qi = Mr[j][i] * xi[i] + Mi[j][i] * xr[i];
-      vr[j] = vr[j] - qr;
-      vi[j] = vi[j] - qi;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < NS; i++) fin = fin && cedm::isfinite_(xr[i]) && cedm::isfinite_(xi[i]);
-  if (!fin) { atomicOr(&B.sweep_fail[c], 1); return; }
-  double gr = 0.0, gi = 0.0;
-#pragma unroll
-  for (int i = 0; i < NS; i++) {
-    const double k = (double)gg[i];
-    gr = cedm::fma_(k, xr[i], gr);
-    gi = cedm::fma_(k, xi[i], gi);
-  }
-  const double pr = tab<double>(P, P->off_pr)[w], pim = tab<double>(P, P->off_pi)[w];
-  const double rr = pr * gr - pim * gi, ri = pr * gi + pim * gr;
-  const double m = cedm::hypot(rr, ri);
-  const double raw = cedm::atan2(ri, rr);
-  const double Ps = P->alpha * tab<double>(P, P->off_pabs)[w];
-  const double Cs = cedm::hypot(gr, gi) / P->alpha;
-  const double num = cedm::hypot(1.0 + rr, ri);
-  const double b = num / (cedm::sqrt_(1.0 + Ps * Ps) * cedm::sqrt_(1.0 + Cs * Cs));
   double* out = B.mag_raw + ((size_t)c * F + w) * 2;
   out[0] = m;
   out[1] = raw;
@@ -765,40 +432,8 @@ __global__ __launch_bounds__(64) void k_scan_fp64(const ProblemDev* __restrict__
 // ------------------------------------------------------------------------------------------------
 // Launch helpers (host side of this translation unit)
 
-using SimFn = void (*)(const ProblemDev*, StageBufs);
-
-template <int NS, int ND>
-static void launch_sim_t(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  hipLaunchKernelGGL((k_sim_fp64<NS, ND>), dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
-}
-
-template <int NS>
-static bool launch_sim_ns(int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  switch (nd) {
-    case 1: launch_sim_t<NS, 1>(P, B, st); return true;
-    case 2: launch_sim_t<NS, 2>(P, B, st); return true;
-    case 3: launch_sim_t<NS, 3>(P, B, st); return true;
-    case 4: launch_sim_t<NS, 4>(P, B, st); return true;
-    case 5: launch_sim_t<NS, 5>(P, B, st); return true;
-    case 6: launch_sim_t<NS, 6>(P, B, st); return true;
-    case 7: launch_sim_t<NS, 7>(P, B, st); return true;
-    case 8: launch_sim_t<NS, 8>(P, B, st); return true;
-  }
-  return false;
-}
-
 bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  switch (ns) {
-    case 1: return launch_sim_ns<1>(nd, P, B, st);
-    case 2: return launch_sim_ns<2>(nd, P, B, st);
-    case 3: return launch_sim_ns<3>(nd, P, B, st);
-    case 4: return launch_sim_ns<4>(nd, P, B, st);
-    case 5: return launch_sim_ns<5>(nd, P, B, st);
-    case 6: return launch_sim_ns<6>(nd, P, B, st);
-    case 7: return launch_sim_ns<7>(nd, P, B, st);
-    case 8: return launch_sim_ns<8>(nd, P, B, st);
-  }
-  return false;
+  return launch_sim<double>(ns, nd, P, B, B.ybuf, st);
 }
 
 void launch_eig_fp64(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
@@ -811,11 +446,12 @@ void launch_compact(const StageBufs& B, int all, hipStream_t st) {
 
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int nb = (2 * nt + 7) / 8;
+  const dim3 grid((B.n + 3) / 4);
   switch (nb) {
-    case 1: hipLaunchKernelGGL(k_fit_fp64<1>, dim3((B.n + FitShape<1>::CPW - 1) / FitShape<1>::CPW), dim3(64), 0, st, P, B); return true;
-    case 2: hipLaunchKernelGGL(k_fit_fp64<2>, dim3((B.n + FitShape<2>::CPW - 1) / FitShape<2>::CPW), dim3(64), 0, st, P, B); return true;
-    case 3: hipLaunchKernelGGL(k_fit_fp64<3>, dim3((B.n + FitShape<3>::CPW - 1) / FitShape<3>::CPW), dim3(64), 0, st, P, B); return true;
-    case 4: hipLaunchKernelGGL(k_fit_fp64<4>, dim3((B.n + FitShape<4>::CPW - 1) / FitShape<4>::CPW), dim3(64), 0, st, P, B); return true;
+    case 1: hipLaunchKernelGGL(k_fit_fp64<1>, grid, dim3(64), 0, st, P, B); return true;
+    case 2: hipLaunchKernelGGL(k_fit_fp64<2>, grid, dim3(64), 0, st, P, B); return true;
+    case 3: hipLaunchKernelGGL(k_fit_fp64<3>, grid, dim3(64), 0, st, P, B); return true;
+    case 4: hipLaunchKernelGGL(k_fit_fp64<4>, grid, dim3(64), 0, st, P, B); return true;
   }
   return false;
 }

@@ -1,15 +1,448 @@
-// Mixed-precision stage chain (float simulation, fit accumulation and sweep; double eigensolve,
-// normal-equation solve and crossover refinement).
+// Mixed-precision stage kernels for gfx950.
+//
+//   simulation     float state (generic: k_sim<float,NS,ND>, or the specialised kernel); the fit
+//                  window of y is stored as float, SoA.
+//   modal fit      k_fitcf: the normal equations of the damped-sinusoid fit (reference
+//                  src/modalsetup.cu:64-91) are formed in double without a pass over time for the
+//                  Gram: each entry sum_t e^{(s_e+s_f) tau} trig(w_e tau) trig(w_f tau) is a geometric
+//                  series, summed in closed form with expm1 (no cancellation for slow modes).  The
+//                  right-hand side sum_t phi(t)(y(t) - yss) runs over the stored trajectory with the
+//                  basis advanced by a double complex recurrence.  Then the same LDL^T solve and
+//                  score as the FP64 chain (src/modalsolve.cu).
+//   margin sweep   k_sweep32: every frequency in float.  k_mark32: per candidate, the intervals whose
+//                  float values show a gain or phase crossing, or lie within a guard band of one,
+//                  and the point of smallest b, are marked.  k_refine64: marked points are recomputed
+//                  with the FP64 sweep point (the FP64 chain's arithmetic).  k_scan_mixed: the
+//                  reference's unwrap / crossing / interpolation scan (src/allmargin.cu:119-180) in
+//                  double over the refined values.
 #include <hip/hip_runtime.h>
 
 #include <string>
 
+#include "ce_fitsolve_dev.h"
+#include "ce_sim_dev.h"
 #include "ce_stage.h"
+#include "ce_sweep_dev.h"
 
 namespace ce {
 
-bool launch_mixed_chain(const ce_config&, const ProblemDev*, const StageBufs&, hipStream_t, hipEvent_t*, std::string& err) {
-  err = "mixed precision chain not built yet";
+constexpr float GUARD_MAG = 1e-3f;   // |m - 1| below this: refine in FP64
+constexpr float GUARD_PHASE = 1e-3f; // phase within this of an odd multiple of pi, or a jump within
+                                     // this of pi: refine in FP64
+
+// ------------------------------------------------------------------------------------------------
+struct Cplx {
+  double r, i;
+};
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx b) { return Cplx{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ Cplx cdiv(Cplx a, Cplx b) {
+  const double d = b.r * b.r + b.i * b.i;
+  return Cplx{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+// e^{a} - 1 for complex a, accurate near 0
+__device__ __forceinline__ Cplx cexpm1(double ar, double ai) {
+  double s, c;
+  cedm::sincos(ai, s, c);
+  double sh, ch;
+  cedm::sincos(0.5 * ai, sh, ch);
+  const double em = cedm::expm1(ar);
+  return Cplx{em * c - 2.0 * sh * sh, (em + 1.0) * s};
+}
+
+// sum_{t=0}^{N-1} exp((sig + i beta)(tau0 + t ts) - shift)
+__device__ __forceinline__ Cplx geo_sum(double sig, double beta, double tau0, double ts, int N, double shift) {
+  const double ar = sig * ts, ai = beta * ts;
+  Cplx ratio;
+  if (ar == 0.0 && ai == 0.0) {
+    ratio = Cplx{(double)N, 0.0};
+  } else {
+    ratio = cdiv(cexpm1((double)N * ar, (double)N * ai), cexpm1(ar, ai));
+  }
+  const double mag = cedm::exp(sig * tau0 - shift);
+  double s, c;
+  cedm::sincos(beta * tau0, s, c);
+  return cmul(Cplx{mag * c, mag * s}, ratio);
+}
+
+template <int NB>
+struct CfShape {
+  static constexpr int G = 16;
+  static constexpr int CPW = 4;
+  static constexpr int NCP = 8 * NB;
+  static constexpr int NTRI = NCP * (NCP + 1) / 2;
+};
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, StageBufs B) {
+  using S = CfShape<NB>;
+  constexpr int G = S::G, CPW = S::CPW, NTRI = S::NTRI, NCP = S::NCP;
+  __shared__ double gram[CPW][NTRI];
+  __shared__ double vec[CPW][5][NCP];
+  __shared__ double part_sum[CPW][G][4];
+  const int lane = threadIdx.x;
+  const int slot = lane / G, g = lane % G;
+  const int idx = blockIdx.x * CPW + slot;
+  const int n = B.n;
+  const int nt = P->nt, nc2 = 2 * nt;
+  bool active = false;
+  int u = 0, c = 0;
+  double* row = nullptr;
+  if (idx < *B.count) {
+    c = B.list[idx];
+    row = B.info + (size_t)c * P->info_w;
+    const bool stable = (B.flags[c] & F_UNSTABLE) == 0;
+    active = stable && (row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)));
+    u = B.uniq[c];
+  }
+  for (int k = g; k < NTRI; k += G) gram[slot][k] = 0.0;
+  for (int k = g; k < NCP; k += G) vec[slot][0][k] = 0.0;
+  __syncthreads();
+  const double ts = P->ts, t0 = P->fit_t0, t1 = P->fit_t1;
+  const int f0 = P->fit_start_id, N = P->fit_count;
+  const double tau0 = (double)f0 * ts;
+  if (active) {
+    // right-hand side sum_t phi(t)(y(t) - yss): lanes split as (mode, time part) so that all G
+    // lanes work when u < G; each part restarts the basis at its first sample
+    int sp = 1;
+    while (sp * 2 * (u > 0 ? u : 1) <= G) sp *= 2;
+    const int e = g / sp, part = g % sp;
+    double ayr = 0.0, ayi = 0.0, asr = 0.0, asi = 0.0;
+    if (e < u) {
+      const double re = B.mode_re[(size_t)e * n + c], im = B.mode_im[(size_t)e * n + c];
+      const double sh = cedm::fmax_(re * t0, re * t1);
+      const int chunk = (N + sp - 1) / sp;
+      const int tb = part * chunk, te = (tb + chunk < N) ? tb + chunk : N;
+      const double tau = (double)(f0 + tb) * ts;
+      const double m0 = cedm::exp(re * tau - sh);
+      double s0, c0;
+      cedm::sincos(im * tau, s0, c0);
+      double br = m0 * c0, bi = m0 * s0;
+      const double ms = cedm::exp(re * ts);
+      double s1, c1;
+      cedm::sincos(im * ts, s1, c1);
+      const double zr = ms * c1, zi = ms * s1;
+      const float* ycol = B.ybuf32 + idx;
+#pragma unroll 8
+      for (int t = tb; t < te; t++) {
+        const double y = (double)ycol[(size_t)t * n];
+        ayr = __builtin_fma(br, y, ayr);
+        ayi = __builtin_fma(bi, y, ayi);
+        asr = asr + br;
+        asi = asi + bi;
+        const double nr = br * zr - bi * zi;
+        bi = br * zi + bi * zr;
+        br = nr;
+      }
+    }
+    part_sum[slot][g][0] = ayr;
+    part_sum[slot][g][1] = ayi;
+    part_sum[slot][g][2] = asr;
+    part_sum[slot][g][3] = asi;
+    // Gram in closed form: mode pairs (e >= f) spread over the group's lanes
+    const int npairs = u * (u + 1) / 2;
+    for (int p = g; p < npairs; p += G) {
+      int ea = 0;
+      while ((ea + 1) * (ea + 2) / 2 <= p) ea++;
+      const int fb = p - ea * (ea + 1) / 2;
+      const double se = B.mode_re[(size_t)ea * n + c], we = B.mode_im[(size_t)ea * n + c];
+      const double sf = B.mode_re[(size_t)fb * n + c], wf = B.mode_im[(size_t)fb * n + c];
+      const double shift = cedm::fmax_(se * t0, se * t1) + cedm::fmax_(sf * t0, sf * t1);
+      const Cplx sdif = geo_sum(se + sf, we - wf, tau0, ts, N, shift);
+      const Cplx ssum = geo_sum(se + sf, we + wf, tau0, ts, N, shift);
+      const double cc = 0.5 * (sdif.r + ssum.r), ss = 0.5 * (sdif.r - ssum.r);
+      const double cs = 0.5 * (ssum.i - sdif.i), sc = 0.5 * (ssum.i + sdif.i);
+      gram[slot][tri(2 * ea, 2 * fb)] = cc;
+      gram[slot][tri(2 * ea + 1, 2 * fb + 1)] = ss;
+      gram[slot][tri(2 * ea + 1, 2 * fb)] = sc;
+      if (ea != fb) gram[slot][tri(2 * ea, 2 * fb + 1)] = cs;
+    }
+  }
+  __syncthreads();
+  if (active && g < G / 2) {
+    int sp = 1;
+    while (sp * 2 * (u > 0 ? u : 1) <= G) sp *= 2;
+    const int e = g;
+    if (e < u) {
+      double ayr = 0.0, ayi = 0.0, asr = 0.0, asi = 0.0;
+      for (int q = 0; q < sp; q++) {
+        ayr = ayr + part_sum[slot][e * sp + q][0];
+        ayi = ayi + part_sum[slot][e * sp + q][1];
+        asr = asr + part_sum[slot][e * sp + q][2];
+        asi = asi + part_sum[slot][e * sp + q][3];
+      }
+      const double yss = row[P->info_freqresp + 3];
+      vec[slot][0][2 * e] = ayr - yss * asr;
+      vec[slot][0][2 * e + 1] = ayi - yss * asi;
+    }
+  }
+  if (active && g >= G / 2 && g < u) {
+    // modes G/2.. (u > G/2 means one time part per mode)
+    const int e = g;
+    const double yss = row[P->info_freqresp + 3];
+    vec[slot][0][2 * e] = part_sum[slot][e][0] - yss * part_sum[slot][e][2];
+    vec[slot][0][2 * e + 1] = part_sum[slot][e][1] - yss * part_sum[slot][e][3];
+  }
+  __syncthreads();
+  if (!active || g != 0) return;
+  double* G2 = gram[slot];
+  double* atb = vec[slot][0];
+  if (B.diag && B.d_ata) {
+    float* da = B.d_ata + (size_t)c * nc2 * nc2;
+    for (int i = 0; i < nc2; i++)
+      for (int j = 0; j < nc2; j++) da[i * nc2 + j] = (j <= i) ? (float)G2[tri(i, j)] : 0.0f;
+  }
+  if (B.diag && B.d_atb) {
+    float* db = B.d_atb + (size_t)c * nc2;
+    for (int i = 0; i < nc2; i++) db[i] = (float)atb[i];
+  }
+  if (row[P->info_scoring] != 0.0) return;
+  solve_and_score(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4]);
+}
+
+// ------------------------------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(64) void k_sweep32(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x;
+  const int w = blockIdx.y * 64 + threadIdx.x;
+  if (margin_skip(P, B, c)) return;
+  const int F = P->F;
+  if (w >= F) return;
+  if (!tab<int32_t>(P, P->off_pok)[w]) { atomicOr(&B.sweep_fail[c], 1); return; }
+  float m, raw, b;
+  const bool ok = sweep_point<NS, float>(P, B.open_a + (size_t)c * NS * NS, B.gains + (size_t)c * P->cuda_w, w, m, raw, b);
+  float* out = B.mr32 + ((size_t)c * F + w) * 3;
+  out[0] = ok ? m : __builtin_nanf("");
+  out[1] = ok ? raw : __builtin_nanf("");
+  out[2] = ok ? b : __builtin_nanf("");
+}
+
+__device__ __forceinline__ bool near_odd_pi(float ph) {
+  const float x = ph + (float)PI_D;
+  const float k = __builtin_rintf(x / (float)TWO_PI_D);
+  return __builtin_fabsf(x - k * (float)TWO_PI_D) < GUARD_PHASE;
+}
+
+// Marks, per candidate, the sweep points to recompute in FP64 (one lane per candidate).
+__global__ __launch_bounds__(64) void k_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= B.n) return;
+  if (margin_skip(P, B, c) || B.sweep_fail[c]) return;
+  const int F = P->F;
+  const float* mr = B.mr32 + (size_t)c * F * 3;
+  int8_t* rf = B.refined + (size_t)c * F;
+  int nmark = 0;
+  bool overflow = false;
+  int* mine = B.rslots + (size_t)c * RMAX;
+  auto mark = [&](int w) {
+    if (rf[w] != 0) return;
+    if (nmark >= RMAX) { overflow = true; return; }
+    rf[w] = 1;
+    mine[nmark++] = w;
+  };
+  const float PIf = (float)PI_D, TPf = (float)TWO_PI_D;
+  float prevMag = 0.0f, prevPhase = 0.0f, prevRaw = 0.0f, bestb = __builtin_inff();
+  int argb = -1;
+  for (int w = 0; w < F; w++) {
+    const float m = mr[3 * w], raw = mr[3 * w + 1], b = mr[3 * w + 2];
+    const bool fin = __builtin_isfinite(m) && __builtin_isfinite(raw);
+    if (b < bestb) { bestb = b; argb = w; }
+    if (w > 0) {
+      float delta = raw - prevRaw;
+      const bool amb = __builtin_fabsf(__builtin_fabsf(delta) - PIf) < GUARD_PHASE;
+      if (delta > PIf) delta -= TPf;
+      else if (delta < -PIf) delta += TPf;
+      const float ph = prevPhase + delta;
+      const bool gcross = (prevMag - 1.0f) * (m - 1.0f) <= 0.0f;
+      const bool gnear = __builtin_fabsf(prevMag - 1.0f) < GUARD_MAG || __builtin_fabsf(m - 1.0f) < GUARD_MAG;
+      const float ka = __builtin_floorf((prevPhase + PIf) / TPf), kb = __builtin_floorf((ph + PIf) / TPf);
+      const bool pcross = ka != kb;
+      const bool pnear = near_odd_pi(prevPhase) || near_odd_pi(ph);
+      if (gcross || gnear || pcross || pnear || amb || !fin) {
+        mark(w - 1);
+        mark(w);
+      }
+      prevPhase = ph;
+    } else {
+      prevPhase = raw;
+      if (!fin) mark(0);
+    }
+    prevMag = m;
+    prevRaw = raw;
+  }
+  // the smallest b(w) and every point whose float b lies within a band of it
+  if (argb >= 0) {
+    mark(argb);
+    const float band = bestb + (1e-3f + 1e-2f * bestb);
+    for (int w = 0; w < F; w++)
+      if (mr[3 * w + 2] <= band) mark(w);
+  }
+  B.rcnt[c] = nmark;
+  if (overflow) atomicOr(&B.flags[c], F_REFINE_OVERFLOW);
+}
+
+// Gathers the per-candidate refinement slots into one dense list: one atomic per wave.
+__global__ __launch_bounds__(64) void k_rgather(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  int cnt = 0;
+  if (c < B.n && !margin_skip(P, B, c) && !B.sweep_fail[c]) cnt = B.rcnt[c];
+  // inclusive scan of cnt over the wave
+  int incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d);
+    if ((int)threadIdx.x >= d) incl += v;
+  }
+  const int total = __shfl(incl, 63);
+  int base = 0;
+  if (threadIdx.x == 63 && total > 0) base = atomicAdd(B.rcount, total);
+  base = __shfl(base, 63);
+  const int start = base + incl - cnt;
+  for (int k = 0; k < cnt; k++) B.rlist[start + k] = make_int2(c, B.rslots[(size_t)c * RMAX + k]);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int total = *B.rcount;
+  const int F = P->F;
+  for (int i = blockIdx.x * 64 + threadIdx.x; i < total; i += gridDim.x * 64) {
+    const int2 e = B.rlist[i];
+    const int c = e.x, w = e.y;
+    double m, raw, b;
+    const bool ok = sweep_point<NS, double>(P, B.open_a + (size_t)c * NS * NS, B.gains + (size_t)c * P->cuda_w, w, m, raw, b);
+    double* out = B.mag_raw + ((size_t)c * F + w) * 3;
+    out[0] = m;
+    out[1] = raw;
+    out[2] = b;
+    B.refined[(size_t)c * F + w] = ok ? 2 : 3;
+  }
+}
+
+// Crossing scan over the refined values (src/allmargin.cu:119-180), one candidate per lane.
+__global__ __launch_bounds__(64) void k_scan_mixed(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= B.n) return;
+  double* row = B.info + (size_t)c * P->info_w;
+  double* sc = row + P->info_scoring;
+  double* am = row + P->info_allmargin;
+  const double UE = P->unstable_err, EF = P->eig_fail_err;
+  const int flags = B.flags[c];
+  if (sc[0] >= UE) {
+    sc[1] = sc[0]; sc[2] = sc[0];
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    return;
+  }
+  if (flags & F_NOMINAL_UNSTABLE) {
+    sc[1] = sc[0];
+    sc[2] = UE;
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    const double total = cedm::log((sc[0] + UE) + EPS30);
+    sc[0] = cedm::isfinite_(total) ? total : EF;
+    return;
+  }
+  const int F = P->F;
+  const double ts = P->ts;
+  const double* th = tab<double>(P, P->off_theta);
+  const float* mr = B.mr32 + (size_t)c * F * 3;
+  const double* m64 = B.mag_raw + (size_t)c * F * 3;
+  const int8_t* rf = B.refined + (size_t)c * F;
+  bool fail = B.sweep_fail[c] != 0;
+  double PM = FLTMAX, GM = FLTMAX, DM = FLTMAX, Bmin = 1.0;
+  bool crossed = false;
+  double prevMag = 0.0, prevPhase = 0.0, prevRaw = 0.0, prevTheta = 0.0;
+  for (int w = 0; w < F && !fail; w++) {
+    double m, raw, b;
+    const int r = rf[w];
+    if (r == 3) { fail = true; break; }
+    if (r == 2) { m = m64[3 * w]; raw = m64[3 * w + 1]; b = m64[3 * w + 2]; }
+    else { m = (double)mr[3 * w]; raw = (double)mr[3 * w + 1]; b = (double)mr[3 * w + 2]; }
+    if (!(cedm::isfinite_(m) && cedm::isfinite_(raw))) { fail = true; break; }
+    Bmin = cedm::fmin_(Bmin, b);
+    const double theta_w = th[w];
+    if (w > 0) {
+      double delta = raw - prevRaw;
+      if (delta > PI_D) delta = delta - TWO_PI_D;
+      else if (delta < -PI_D) delta = delta + TWO_PI_D;
+      const double ph = prevPhase + delta;
+      if ((prevMag - 1.0) * (m - 1.0) <= 0.0 && m != prevMag) {
+        const double frac = (1.0 - prevMag) / (m - prevMag);
+        const double phase_c = prevPhase + frac * (ph - prevPhase);
+        crossed = true;
+        double pm_deg = 180.0 + phase_c * RAD2DEG;
+        pm_deg = pm_deg - 360.0 * cedm::floor_((pm_deg + 180.0) / 360.0);
+        PM = cedm::fmin_(PM, pm_deg);
+        const double theta_c = prevTheta + frac * (theta_w - prevTheta);
+        const double pm_rad = pm_deg * DEG2RAD;
+        if (theta_c > EPS6 && pm_rad >= 0.0) DM = cedm::fmin_(DM, ((pm_rad / theta_c) * ts) * 1000.0);
+      }
+      const double a = prevPhase + PI_D, bb = ph + PI_D;
+      const double ka = cedm::floor_(a / TWO_PI_D), kb = cedm::floor_(bb / TWO_PI_D);
+      if (ka != kb && bb != a) {
+        const double target = TWO_PI_D * (bb > a ? kb : ka);
+        const double frac = (target - a) / (bb - a);
+        const double mag_p = prevMag + frac * (m - prevMag);
+        GM = cedm::fmin_(GM, -20.0 * cedm::log10(mag_p));
+      }
+      prevPhase = ph;
+    } else {
+      prevPhase = raw;
+    }
+    prevMag = m;
+    prevRaw = raw;
+    prevTheta = theta_w;
+  }
+  if (fail) {
+    sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
+    am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+    return;
+  }
+  const double err_gm = cedm::fmax_(0.0, P->target_gm - GM) / P->target_gm;
+  const double err_pm = crossed ? cedm::fmax_(0.0, P->target_pm - PM) / P->target_pm : 0.0;
+  const double err_b = cedm::fmax_(0.0, P->target_b - Bmin) / P->target_b;
+  const double margin = (err_gm * err_gm + err_pm * err_pm) + err_b * err_b;
+  sc[1] = sc[0];
+  sc[2] = margin;
+  am[0] = GM; am[1] = PM; am[2] = DM; am[3] = 1.0; am[4] = Bmin;
+  const double total = cedm::log((sc[0] + margin) + EPS30);
+  sc[0] = cedm::isfinite_(total) ? total : EF;
+}
+
+// ------------------------------------------------------------------------------------------------
+bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  return launch_sim<float>(ns, nd, P, B, B.ybuf32, st);
+}
+
+bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  const int nb = (2 * nt + 7) / 8;
+  const dim3 grid((B.n + 3) / 4);
+  switch (nb) {
+    case 1: hipLaunchKernelGGL(k_fitcf<1>, grid, dim3(64), 0, st, P, B); return true;
+    case 2: hipLaunchKernelGGL(k_fitcf<2>, grid, dim3(64), 0, st, P, B); return true;
+    case 3: hipLaunchKernelGGL(k_fitcf<3>, grid, dim3(64), 0, st, P, B); return true;
+    case 4: hipLaunchKernelGGL(k_fitcf<4>, grid, dim3(64), 0, st, P, B); return true;
+  }
+  return false;
+}
+
+template <int NS>
+static void launch_sweep_mixed_t(int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_sweep32<NS>, dim3(B.n, (F + 63) / 64), dim3(64), 0, st, P, B);
+  hipLaunchKernelGGL(k_mark32, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  hipLaunchKernelGGL(k_rgather, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
+  hipLaunchKernelGGL(k_scan_mixed, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+}
+
+bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  switch (ns) {
+    case 1: launch_sweep_mixed_t<1>(F, P, B, st); return true;
+    case 2: launch_sweep_mixed_t<2>(F, P, B, st); return true;
+    case 3: launch_sweep_mixed_t<3>(F, P, B, st); return true;
+    case 4: launch_sweep_mixed_t<4>(F, P, B, st); return true;
+    case 5: launch_sweep_mixed_t<5>(F, P, B, st); return true;
+    case 6: launch_sweep_mixed_t<6>(F, P, B, st); return true;
+    case 7: launch_sweep_mixed_t<7>(F, P, B, st); return true;
+    case 8: launch_sweep_mixed_t<8>(F, P, B, st); return true;
+  }
   return false;
 }
 

@@ -1,0 +1,219 @@
+// Generic (interpreted) closed-loop simulation kernel, reference src/discrete_solves.cu:12-79,
+// templated on the arithmetic type (double: FP64 chain; float: mixed chain) and on the observer /
+// plant dimensions.  One candidate per lane; the state lives in VGPRs; plant and observer matrices
+// and the decoded instruction chains are wave-uniform (scalar loads).  Used when runtime
+// specialisation is disabled and for batches of uneven problems.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ce_stage.h"
+
+namespace ce {
+
+template <typename R>
+struct RealOps;
+template <>
+struct RealOps<double> {
+  __device__ static double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+  __device__ static double fmin(double a, double b) { return __builtin_fmin(a, b); }
+  __device__ static double fmax(double a, double b) { return __builtin_fmax(a, b); }
+  __device__ static double inf() { return __builtin_inf(); }
+  __device__ static double eps6() { return EPS6; }
+  __device__ static double aval(const ProblemDev* P, const Ins& in, int which) { return which ? in.bval : in.aval; }
+  __device__ static const double* sai(const ProblemDev* P) { return P->sai; }
+  __device__ static const double* sb(const ProblemDev* P) { return P->sb; }
+  __device__ static const double* sc(const ProblemDev* P) { return P->sc; }
+  __device__ static double sd(const ProblemDev* P) { return P->sd; }
+  __device__ static const double* dai(const ProblemDev* P) { return P->dai; }
+  __device__ static const double* dbu(const ProblemDev* P) { return P->dbu; }
+  __device__ static const double* dbw(const ProblemDev* P) { return P->dbw; }
+  __device__ static double amp(const ProblemDev* P) { return P->ref_amp; }
+};
+template <>
+struct RealOps<float> {
+  __device__ static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+  __device__ static float fmin(float a, float b) { return __builtin_fminf(a, b); }
+  __device__ static float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+  __device__ static float inf() { return __builtin_inff(); }
+  __device__ static float eps6() { return (float)EPS6; }
+  __device__ static float aval(const ProblemDev* P, const Ins& in, int which) {
+    return which ? (float)in.bval : (float)in.aval;
+  }
+  __device__ static const float* sai(const ProblemDev* P) { return P->fsai; }
+  __device__ static const float* sb(const ProblemDev* P) { return P->fsb; }
+  __device__ static const float* sc(const ProblemDev* P) { return P->fsc; }
+  __device__ static float sd(const ProblemDev* P) { return P->fsd; }
+  __device__ static const float* dai(const ProblemDev* P) { return P->fdai; }
+  __device__ static const float* dbu(const ProblemDev* P) { return P->fdbu; }
+  __device__ static const float* dbw(const ProblemDev* P) { return P->fdbw; }
+  __device__ static float amp(const ProblemDev* P) { return P->fref_amp; }
+};
+
+// Instruction chain (reference src/main.cu:260-291).  ops/count/col are wave-uniform.
+template <typename R, int DIM, int MAXCOUNT, bool HAS_SEED>
+__device__ __forceinline__ R interp_dev(const Ins* __restrict__ ops, int count, const R (&src)[DIM], R seed) {
+  using O = RealOps<R>;
+  R result = R(0);
+  bool started = false;
+#pragma unroll
+  for (int r = 0; r < MAXCOUNT; r++) {
+    if (r >= count) break;
+    const Ins in = ops[r];
+    if (in.op == 0) continue;
+    R pos = src[0];
+#pragma unroll
+    for (int k = 1; k < DIM; k++)
+      if (in.col == k) pos = src[k];
+    R prev;
+    if (started) prev = result;
+    else if (HAS_SEED) prev = (seed == seed) ? seed : pos;
+    else prev = pos;
+    const R av = (R)in.aval, bv = (R)in.bval;
+    const R a = in.akind == K_CONST ? av : (in.akind == K_STATE ? pos : prev);
+    const R b = in.bkind == K_CONST ? bv : (in.bkind == K_STATE ? pos : prev);
+    switch (in.op) {
+      case 1: result = a + b; break;
+      case 2: result = a - b; break;
+      case 3: result = a * b; break;
+      case 4: result = a / b; break;
+      case 5: result = O::fmin(O::fmax(prev, a), b); break;
+      case 6: result = (R)((R(0) < a) - (a < R(0))); break;
+      default: break;
+    }
+    started = true;
+  }
+  return result;
+}
+
+// Closed-loop simulation over the compacted candidate list.  Stores the fit window of y
+// (SoA: ybuf[(t - fit_start) * n + slot]) and the steady-state mean and overshoot of the INFO row
+// (reference src/modalsetup.cu:52-62).  max_t fl(y_t yss) equals fl(max_t y_t * yss) for yss > 0 and
+// fl(min_t y_t * yss) for yss < 0, since rounding is monotonic.
+template <typename R, int NS, int ND>
+__global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, StageBufs B, R* __restrict__ ybuf_base) {
+  using O = RealOps<R>;
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= *B.count) return;
+  const int c = B.list[idx];
+  const bool unstable = (B.flags[c] & F_UNSTABLE) != 0;
+  const int n = B.n;
+  const float* g = B.gains + (size_t)c * P->cuda_w;
+  R K[NS], L[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) { K[i] = (R)g[i]; L[i] = (R)g[NS + i]; }
+  const R N = (R)g[2 * NS];
+  R xh[NS], xp[ND];
+#pragma unroll
+  for (int i = 0; i < NS; i++) xh[i] = R(0);
+#pragma unroll
+  for (int i = 0; i < ND; i++) xp[i] = R(0);
+  const int T = P->T, f0 = P->fit_start_id, f1 = P->fit_start_id + P->fit_count;
+  const int s0 = P->steady_start_id, k0 = P->ref_step_id, nw = P->nw;
+  const int dc_n = P->dc_n, u_n = P->u_n;
+  const R amp = O::amp(P), sd = O::sd(P);
+  const R* sai = O::sai(P);
+  const R* sb = O::sb(P);
+  const R* scv = O::sc(P);
+  const R* dai = O::dai(P);
+  const R* dbu = O::dbu(P);
+  const R* dbw = O::dbw(P);
+  R ysum = R(0), ymax = -O::inf(), ymin = O::inf();
+  R* ybuf = ybuf_base + idx;
+  float* dy = (B.diag && B.d_y) ? B.d_y + (size_t)c * T : nullptr;
+  for (int t = 0; t < T; t++) {
+    const R y = interp_dev<R, ND, CE_MAX_DC_INSTRUCTIONS, false>(P->dc_ops, dc_n, xp, R(0));
+    if (dy) dy[t] = (float)y;
+    if (t >= f0 && t < f1) ybuf[(size_t)(t - f0) * n] = y;
+    if (t >= s0) ysum = ysum + y;
+    ymax = O::fmax(ymax, y);
+    ymin = O::fmin(ymin, y);
+    R kx = R(0);
+#pragma unroll
+    for (int k = 0; k < NS; k++) kx = O::fma(K[k], xh[k], kx);
+    const R r = (t < k0) ? R(0) : amp;
+    const R u_raw = O::fma(N, r, -kx);
+    const R u = interp_dev<R, NS, CE_MAX_U_INSTRUCTIONS, true>(P->u_ops, u_n, xh, u_raw);
+    R cx = R(0);
+#pragma unroll
+    for (int k = 0; k < NS; k++) cx = O::fma(scv[k], xh[k], cx);
+    const R yerr = y - O::fma(sd, u, cx);
+    R dh[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      R acc = O::fma(L[i], yerr, sb[i] * u);
+#pragma unroll
+      for (int k = 0; k < NS; k++) acc = O::fma(sai[i * NS + k], xh[k], acc);
+      dh[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++) xh[i] = xh[i] + dh[i];
+    R nls[ND + 1];
+#pragma unroll
+    for (int i = 0; i < ND; i++) nls[i] = xp[i];
+    nls[ND] = y;
+    R W[MAXW];
+#pragma unroll
+    for (int q = 0; q < MAXW; q++) {
+      W[q] = R(0);
+      if (q < nw) W[q] = interp_dev<R, ND + 1, CE_MAX_NL_INSTRUCTIONS, false>(P->nl_ops[q], P->nl_n[q], nls, R(0));
+    }
+    R ph[ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++) {
+      R base = dbu[i] * u;
+#pragma unroll
+      for (int q = 0; q < MAXW; q++)
+        if (q < nw) base = O::fma(dbw[i * nw + q], W[q], base);
+      R acc = base;
+#pragma unroll
+      for (int k = 0; k < ND; k++) acc = O::fma(dai[i * ND + k], xp[k], acc);
+      ph[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < ND; i++) xp[i] = xp[i] + ph[i];
+  }
+  if (unstable) return;
+  const R yss = ysum / (R)(T - s0);
+  const R pk = O::fmax(R(0), ((yss < R(0)) ? ymin : ymax) * yss);
+  double* fr = B.info + (size_t)c * P->info_w + P->info_freqresp;
+  fr[2] = (double)(pk / O::fmax(yss * yss, O::eps6()) - R(1));
+  fr[3] = (double)yss;
+}
+
+template <typename R, int NS, int ND>
+static void launch_sim_t(const ProblemDev* P, const StageBufs& B, R* ybuf, hipStream_t st) {
+  hipLaunchKernelGGL((k_sim<R, NS, ND>), dim3((B.n + 63) / 64), dim3(64), 0, st, P, B, ybuf);
+}
+
+template <typename R, int NS>
+static bool launch_sim_ns(int nd, const ProblemDev* P, const StageBufs& B, R* ybuf, hipStream_t st) {
+  switch (nd) {
+    case 1: launch_sim_t<R, NS, 1>(P, B, ybuf, st); return true;
+    case 2: launch_sim_t<R, NS, 2>(P, B, ybuf, st); return true;
+    case 3: launch_sim_t<R, NS, 3>(P, B, ybuf, st); return true;
+    case 4: launch_sim_t<R, NS, 4>(P, B, ybuf, st); return true;
+    case 5: launch_sim_t<R, NS, 5>(P, B, ybuf, st); return true;
+    case 6: launch_sim_t<R, NS, 6>(P, B, ybuf, st); return true;
+    case 7: launch_sim_t<R, NS, 7>(P, B, ybuf, st); return true;
+    case 8: launch_sim_t<R, NS, 8>(P, B, ybuf, st); return true;
+  }
+  return false;
+}
+
+template <typename R>
+static bool launch_sim(int ns, int nd, const ProblemDev* P, const StageBufs& B, R* ybuf, hipStream_t st) {
+  switch (ns) {
+    case 1: return launch_sim_ns<R, 1>(nd, P, B, ybuf, st);
+    case 2: return launch_sim_ns<R, 2>(nd, P, B, ybuf, st);
+    case 3: return launch_sim_ns<R, 3>(nd, P, B, ybuf, st);
+    case 4: return launch_sim_ns<R, 4>(nd, P, B, ybuf, st);
+    case 5: return launch_sim_ns<R, 5>(nd, P, B, ybuf, st);
+    case 6: return launch_sim_ns<R, 6>(nd, P, B, ybuf, st);
+    case 7: return launch_sim_ns<R, 7>(nd, P, B, ybuf, st);
+    case 8: return launch_sim_ns<R, 8>(nd, P, B, ybuf, st);
+  }
+  return false;
+}
+
+}  // namespace ce

@@ -8,8 +8,11 @@ namespace ce {
 enum : int32_t {
   F_UNSTABLE = 1,          // some closed-loop eigenvalue has |lambda| > 1 (reference src/modalsetup.cu:26)
   F_NOMINAL_UNSTABLE = 2,  // nominal loop of the margins stage unstable (reference src/allmargin.cu:61-87)
-  F_NOMINAL_SKIPPED = 4    // nominal check not run (candidate already at or above unstable_err)
+  F_NOMINAL_SKIPPED = 4,   // nominal check not run (candidate already at or above unstable_err)
+  F_REFINE_OVERFLOW = 8    // mixed sweep: more than RMAX points wanted FP64 refinement
 };
+
+constexpr int RMAX = 32;   // FP64 refinement points per candidate in the mixed sweep
 
 struct StageBufs {
   int32_t n;                    // candidates in this launch
@@ -28,6 +31,13 @@ struct StageBufs {
   int32_t* sweep_fail;          // n
   int32_t* list;                // n: candidates that reach the simulation (compacted)
   int32_t* count;               // 1: entries in list
+  // mixed sweep
+  float* mr32;                  // n x F x 3: float |loop|, arg(loop), b(w)
+  int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite
+  int32_t* rslots;              // n x RMAX: frequencies to refine, per candidate
+  int32_t* rcnt;                // n: used slots per candidate
+  int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
+  int32_t* rcount;              // 1
   // diagnostics in the reference dump layout (float, candidate-major), NULL when not requested
   float* d_y;
   float* d_lina;
