@@ -1,0 +1,39 @@
+"""Small hand-built problems for the oracle pipeline tests."""
+import numpy as np
+
+from controllerengine_amd.problem import Problem, encode, info_width
+from controllerengine_amd.workloads import SPEC, Params, row
+
+OP_ADD, OP_MUL, OP_SAT = 1, 3, 5
+STATE, PREV = 0, 35
+
+
+def linear_problem(A_I, B, C, gains, T=200, sat=None, F=64, fit=(0.05, 0.75), steady=0.75, ref_step=0.05,
+                   duration=None, dt=1e-3, bandwidth=1.0):
+    """Plant = observer model; DC: y = C x as instructions when C is a unit vector scaled by 1;
+    U: pass-through or clamp(u, -sat, sat)."""
+    A_I = np.asarray(A_I, dtype=np.float64)
+    n = A_I.shape[0]
+    A = np.eye(n) + A_I
+    P = Params()
+    one = P.const(1.0)
+    col = int(np.argmax(np.abs(C)))
+    assert C[col] == 1.0 and np.count_nonzero(C) == 1
+    dc = [row(n, col, OP_MUL, STATE, one)]
+    if sat is None:
+        u = [row(n, 0, OP_MUL, PREV, one)]
+    else:
+        lo, hi = P.const(-sat), P.const(sat)
+        u = [row(n, 0, OP_SAT, lo, hi)]
+    cfg = dict(statespace_dim=n, discrete_dim=n, nonlin_dim=0, n_params=len(P.vals), input_size=0,
+               dc_instructions=1, u_instructions=1, nl_instructions=[], n_cores=128, n_rounds=1, mode="batch",
+               time=T, duration=duration or T * dt, ref_step_time=ref_step, ref_amplitude=1.0, fit_start=fit[0],
+               fit_end=fit[1], steady_start=steady, bandwidth=bandwidth, freq=F, freq_min=1e-2,
+               dump_diagnostics=False, **SPEC)
+    g = np.asarray(gains, dtype=np.float32).reshape(-1, 2 * n + 1)
+    return Problem(config=cfg, ss_a=np.float32(A), ss_a_i=np.float32(A_I), ss_b=np.float32(B), ss_c=np.float32(C),
+                   ss_d=np.zeros(1, np.float32), d_a=np.float32(A)[None], d_a_i=np.float32(A_I)[None],
+                   d_bu=np.float32(B)[None], d_bw=np.zeros((1, n, 0), np.float32), d_c=np.float32(C)[None],
+                   base_params=P.array()[None], dc_info=np.array(dc, np.int32), u_info=np.array(u, np.int32),
+                   nl_info=np.zeros((0, n + 1), np.int32), gains=g,
+                   info=np.zeros((g.shape[0], info_width(cfg)), np.float32))
