@@ -1,0 +1,365 @@
+// Lane-group real-Schur eigenvalue solver for gfx950 (FP64).
+//
+// One candidate's matrix lives in LDS and is worked on by a group of G lanes (G = 4, 8 or 16, at
+// least the matrix order).  The step order is the one of the per-lane solver (Eigen's
+// EigenSolver<double>: scaling, Householder Hessenberg reduction, Francis double-shift QR with
+// exceptional shifts, read-out; reference call sites src/eigensolve.cu:11 and src/allmargin.cu:72):
+//
+//  * every scalar that steers control flow (Householder coefficients, shifts, deflation tests,
+//    iteration counters) is computed by every lane of the group from the same LDS values, so the
+//    lanes of a group stay convergent;
+//  * every row / column update is split over lanes by its independent index (a left reflector
+//    updates column j in lane j, a right reflector row i in lane i), and each element sees exactly
+//    the operations, in the same order, as in the per-lane solver.
+//
+// The result is therefore bit-identical to the sequential solver.  Groups of one wave may diverge
+// from each other; a group never waits on another.  Phases that read what other lanes of the group
+// wrote are separated by grp_sync(): a workgroup-scope fence, which keeps the compiler from moving
+// LDS accesses across it and waits for this wave's LDS operations.  Kernels using it run one wave
+// per workgroup, and a wave's LDS operations complete in issue order.
+#pragma once
+
+#include <float.h>
+
+#include "ce_common.h"
+
+namespace ce {
+
+__device__ __forceinline__ void grp_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+struct GHh {
+  double e0, e1, tau, beta;
+};
+
+__device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int len) {
+  GHh h;
+  double tail = cedm::fma_(v1, v1, 0.0);
+  if (len == 3) tail = cedm::fma_(v2, v2, tail);
+  if (tail <= DBL_MIN) {
+    h.tau = 0.0;
+    h.beta = c0;
+    h.e0 = 0.0;
+    h.e1 = 0.0;
+  } else {
+    double beta = cedm::sqrt_(cedm::fma_(c0, c0, tail));
+    if (c0 >= 0.0) beta = -beta;
+    const double den = c0 - beta;
+    h.e0 = v1 / den;
+    h.e1 = (len == 3) ? v2 / den : 0.0;
+    h.tau = (beta - c0) / beta;
+    h.beta = beta;
+  }
+  return h;
+}
+
+// Eigenvalues of the n x n matrix T (LDS, row stride S, destroyed) into WR/WI (LDS, n entries).
+// V: LDS scratch of at least n doubles.  l: lane index in the group.  Returns (in every lane)
+// 0 = success, 1 = no convergence within 40 n iterations, 2 = non-finite eigenvalue.
+template <int G>
+__device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l) {
+  constexpr int S = G + 1;
+  double scale = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      const double a = cedm::fabs_(T[i * S + j]);
+      scale = (scale < a) ? a : scale;
+    }
+  grp_sync();
+  if (scale < DBL_MIN) {
+    if (l < n) { WR[l] = 0.0; WI[l] = 0.0; }
+    grp_sync();
+    return 0;
+  }
+  if (l < n)
+    for (int i = 0; i < n; i++) T[i * S + l] = T[i * S + l] / scale;
+  grp_sync();
+
+  // Hessenberg reduction, column by column
+  for (int i = 0; i + 1 < n; i++) {
+    const int rem = n - i - 1;
+    double tail = 0.0;
+    for (int k = 1; k < rem; k++) tail = cedm::fma_(T[(i + 1 + k) * S + i], T[(i + 1 + k) * S + i], tail);
+    const double c0 = T[(i + 1) * S + i];
+    double tau, beta, den = 1.0;
+    const bool flat = tail <= DBL_MIN;
+    if (flat) {
+      tau = 0.0;
+      beta = c0;
+    } else {
+      beta = cedm::sqrt_(cedm::fma_(c0, c0, tail));
+      if (c0 >= 0.0) beta = -beta;
+      den = c0 - beta;
+      tau = (beta - c0) / beta;
+    }
+    grp_sync();
+    if (l >= 1 && l < rem) {
+      V[l - 1] = flat ? 0.0 : T[(i + 1 + l) * S + i] / den;
+      T[(i + 1 + l) * S + i] = 0.0;
+    }
+    if (l == 0) T[(i + 1) * S + i] = beta;
+    grp_sync();
+    if (rem == 1) {
+      if (l < n) {
+        if (l == i + 1) T[(i + 1) * S + i + 1] = T[(i + 1) * S + i + 1] * (1.0 - tau);
+        T[l * S + i + 1] = T[l * S + i + 1] * (1.0 - tau);
+      }
+      grp_sync();
+    } else if (tau != 0.0) {
+      if (l > i && l < n) {  // left: rows i+1..n-1 of column l
+        const int j = l;
+        double tmp = 0.0;
+        for (int k = 0; k < rem - 1; k++) tmp = cedm::fma_(V[k], T[(i + 2 + k) * S + j], tmp);
+        tmp = tmp + T[(i + 1) * S + j];
+        T[(i + 1) * S + j] = T[(i + 1) * S + j] - tau * tmp;
+        for (int k = 0; k < rem - 1; k++) {
+          const double te = tau * V[k];
+          T[(i + 2 + k) * S + j] = T[(i + 2 + k) * S + j] - te * tmp;
+        }
+      }
+      grp_sync();
+      if (l < n) {  // right: columns i+1..n-1 of row l
+        double* row = T + l * S + i + 1;
+        double tmp = 0.0;
+        for (int k = 0; k < rem - 1; k++) tmp = cedm::fma_(row[1 + k], V[k], tmp);
+        tmp = tmp + row[0];
+        row[0] = row[0] - tau * tmp;
+        const double tt = tau * tmp;
+        for (int k = 0; k < rem - 1; k++) row[1 + k] = row[1 + k] - tt * V[k];
+      }
+      grp_sync();
+    }
+  }
+
+  const double eps = DBL_EPSILON;
+  int iu = n - 1, iter = 0, total_iter = 0;
+  const int max_iters = 40 * n;
+  double exshift = 0.0;
+  double norm = 0.0;
+  for (int j = 0; j < n; j++) {
+    const int lim = (n < j + 2) ? n : j + 2;
+    for (int i = 0; i < lim; i++) norm = norm + cedm::fabs_(T[i * S + j]);
+  }
+  double consider_zero = norm * (eps * eps);
+  consider_zero = (consider_zero < DBL_MIN) ? DBL_MIN : consider_zero;
+  if (norm != 0.0) {
+    while (iu >= 0) {
+      int il = iu;
+      while (il > 0) {
+        double s = cedm::fabs_(T[(il - 1) * S + il - 1]) + cedm::fabs_(T[il * S + il]);
+        s = s * eps;
+        s = (s < consider_zero) ? consider_zero : s;
+        if (cedm::fabs_(T[il * S + il - 1]) <= s) break;
+        il--;
+      }
+      if (il == iu) {
+        grp_sync();
+        if (l == 0) {
+          T[iu * S + iu] = T[iu * S + iu] + exshift;
+          if (iu > 0) T[iu * S + iu - 1] = 0.0;
+        }
+        grp_sync();
+        iu--;
+        iter = 0;
+      } else if (il == iu - 1) {
+        const double p = 0.5 * (T[(iu - 1) * S + iu - 1] - T[iu * S + iu]);
+        const double q = p * p + T[iu * S + iu - 1] * T[(iu - 1) * S + iu];
+        const double gq = T[iu * S + iu - 1];
+        grp_sync();
+        if (l == 0) {
+          T[iu * S + iu] = T[iu * S + iu] + exshift;
+          T[(iu - 1) * S + iu - 1] = T[(iu - 1) * S + iu - 1] + exshift;
+        }
+        grp_sync();
+        if (q >= 0.0) {
+          const double z = cedm::sqrt_(cedm::fabs_(q));
+          const double gp = (p >= 0.0) ? p + z : p - z;
+          double c, s;
+          if (gq == 0.0) {
+            c = gp < 0.0 ? -1.0 : 1.0;
+            s = 0.0;
+          } else if (gp == 0.0) {
+            c = 0.0;
+            s = gq < 0.0 ? 1.0 : -1.0;
+          } else if (cedm::fabs_(gp) > cedm::fabs_(gq)) {
+            const double t = gq / gp;
+            double u = cedm::sqrt_(1.0 + t * t);
+            if (gp < 0.0) u = -u;
+            c = 1.0 / u;
+            s = -t * c;
+          } else {
+            const double t = gp / gq;
+            double u = cedm::sqrt_(1.0 + t * t);
+            if (gq < 0.0) u = -u;
+            s = -1.0 / u;
+            c = -t * s;
+          }
+          if (l >= iu - 1 && l < n) {
+            const double x = T[(iu - 1) * S + l], y = T[iu * S + l];
+            T[(iu - 1) * S + l] = c * x - s * y;
+            T[iu * S + l] = s * x + c * y;
+          }
+          grp_sync();
+          if (l <= iu) {
+            const double x = T[l * S + iu - 1], y = T[l * S + iu];
+            T[l * S + iu - 1] = c * x - s * y;
+            T[l * S + iu] = s * x + c * y;
+          }
+          grp_sync();
+          if (l == 0) T[iu * S + iu - 1] = 0.0;
+        }
+        if (l == 0 && iu > 1) T[(iu - 1) * S + iu - 2] = 0.0;
+        grp_sync();
+        iu -= 2;
+        iter = 0;
+      } else {
+        double sh0 = T[iu * S + iu];
+        double sh1 = T[(iu - 1) * S + iu - 1];
+        double sh2 = T[iu * S + iu - 1] * T[(iu - 1) * S + iu];
+        if (iter == 10) {
+          exshift = exshift + sh0;
+          const double s = cedm::fabs_(T[iu * S + iu - 1]) + cedm::fabs_(T[(iu - 1) * S + iu - 2]);
+          grp_sync();
+          if (l <= iu) T[l * S + l] = T[l * S + l] - sh0;
+          grp_sync();
+          sh0 = 0.75 * s;
+          sh1 = 0.75 * s;
+          sh2 = -0.4375 * s * s;
+        }
+        if (iter == 30) {
+          double s = (sh1 - sh0) / 2.0;
+          s = s * s + sh2;
+          if (s > 0.0) {
+            s = cedm::sqrt_(s);
+            if (sh1 < sh0) s = -s;
+            s = s + (sh1 - sh0) / 2.0;
+            s = sh0 - sh2 / s;
+            exshift = exshift + s;
+            grp_sync();
+            if (l <= iu) T[l * S + l] = T[l * S + l] - s;
+            grp_sync();
+            sh0 = sh1 = sh2 = 0.964;
+          }
+        }
+        iter++;
+        total_iter++;
+        if (total_iter > max_iters) return 1;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        int im;
+        for (im = iu - 2; im >= il; --im) {
+          const double tmm = T[im * S + im];
+          const double r = sh0 - tmm;
+          const double s = sh1 - tmm;
+          v0 = (r * s - sh2) / T[(im + 1) * S + im] + T[im * S + im + 1];
+          v1 = T[(im + 1) * S + im + 1] - tmm - r - s;
+          v2 = T[(im + 2) * S + im + 1];
+          if (im == il) break;
+          const double lhs = T[im * S + im - 1] * (cedm::fabs_(v1) + cedm::fabs_(v2));
+          const double rhs =
+              v0 * (cedm::fabs_(T[(im - 1) * S + im - 1]) + cedm::fabs_(tmm) + cedm::fabs_(T[(im + 1) * S + im + 1]));
+          if (cedm::fabs_(lhs) < eps * rhs) break;
+        }
+        for (int k = im; k <= iu - 2; k++) {
+          const bool first = (k == im);
+          double w0, w1, w2;
+          if (first) { w0 = v0; w1 = v1; w2 = v2; }
+          else { w0 = T[k * S + k - 1]; w1 = T[(k + 1) * S + k - 1]; w2 = T[(k + 2) * S + k - 1]; }
+          const GHh h = grp_make_hh(w0, w1, w2, 3);
+          if (h.beta != 0.0) {
+            grp_sync();
+            if (l == 0) {
+              if (first && k > il) T[k * S + k - 1] = -T[k * S + k - 1];
+              else if (!first) T[k * S + k - 1] = h.beta;
+            }
+            if (h.tau != 0.0) {
+              if (l >= k && l < n) {  // left: rows k..k+2 of column l
+                const int j = l;
+                double tmp = cedm::fma_(h.e0, T[(k + 1) * S + j], 0.0);
+                tmp = cedm::fma_(h.e1, T[(k + 2) * S + j], tmp);
+                tmp = tmp + T[k * S + j];
+                T[k * S + j] = T[k * S + j] - h.tau * tmp;
+                T[(k + 1) * S + j] = T[(k + 1) * S + j] - (h.tau * h.e0) * tmp;
+                T[(k + 2) * S + j] = T[(k + 2) * S + j] - (h.tau * h.e1) * tmp;
+              }
+              grp_sync();
+              const int r1 = ((iu < k + 3) ? iu : k + 3) + 1;
+              if (l < r1) {  // right: columns k..k+2 of row l
+                double* row = T + l * S + k;
+                double tmp = cedm::fma_(row[1], h.e0, 0.0);
+                tmp = cedm::fma_(row[2], h.e1, tmp);
+                tmp = tmp + row[0];
+                row[0] = row[0] - h.tau * tmp;
+                const double tt = h.tau * tmp;
+                row[1] = row[1] - tt * h.e0;
+                row[2] = row[2] - tt * h.e1;
+              }
+            }
+            grp_sync();
+          }
+        }
+        {
+          const GHh h = grp_make_hh(T[(iu - 1) * S + iu - 2], T[iu * S + iu - 2], 0.0, 2);
+          if (h.beta != 0.0) {
+            grp_sync();
+            if (l == 0) T[(iu - 1) * S + iu - 2] = h.beta;
+            if (h.tau != 0.0) {
+              if (l >= iu - 1 && l < n) {
+                const int j = l;
+                double tmp = cedm::fma_(h.e0, T[iu * S + j], 0.0);
+                tmp = tmp + T[(iu - 1) * S + j];
+                T[(iu - 1) * S + j] = T[(iu - 1) * S + j] - h.tau * tmp;
+                T[iu * S + j] = T[iu * S + j] - (h.tau * h.e0) * tmp;
+              }
+              grp_sync();
+              if (l < iu + 1) {
+                double* row = T + l * S + iu - 1;
+                double tmp = cedm::fma_(row[1], h.e0, 0.0);
+                tmp = tmp + row[0];
+                row[0] = row[0] - h.tau * tmp;
+                const double tt = h.tau * tmp;
+                row[1] = row[1] - tt * h.e0;
+              }
+            }
+            grp_sync();
+          }
+        }
+        if (l >= im + 2 && l <= iu) {
+          T[l * S + l - 2] = 0.0;
+          if (l > im + 2) T[l * S + l - 3] = 0.0;
+        }
+        grp_sync();
+      }
+    }
+  }
+  grp_sync();
+  if (l < n)
+    for (int i = 0; i < n; i++) T[i * S + l] = T[i * S + l] * scale;
+  grp_sync();
+  int rc = 0;
+  int i = 0;
+  while (i < n) {
+    if (i == n - 1 || T[(i + 1) * S + i] == 0.0) {
+      const double w = T[i * S + i];
+      if (l == 0) { WR[i] = w; WI[i] = 0.0; }
+      if (!cedm::isfinite_(w)) { rc = 2; break; }
+      i++;
+    } else {
+      const double p = 0.5 * (T[i * S + i] - T[(i + 1) * S + i + 1]);
+      double t0 = T[(i + 1) * S + i], t1 = T[i * S + i + 1];
+      const double ap = cedm::fabs_(p), a0 = cedm::fabs_(t0), a1 = cedm::fabs_(t1);
+      const double m01 = (a0 < a1) ? a1 : a0;
+      const double maxval = (ap < m01) ? m01 : ap;
+      t0 = t0 / maxval;
+      t1 = t1 / maxval;
+      const double p0 = p / maxval;
+      const double z = maxval * cedm::sqrt_(cedm::fabs_(p0 * p0 + t0 * t1));
+      const double w = T[(i + 1) * S + i + 1] + p;
+      if (l == 0) { WR[i] = w; WI[i] = z; WR[i + 1] = w; WI[i + 1] = -z; }
+      if (!(cedm::isfinite_(w) && cedm::isfinite_(z))) { rc = 2; break; }
+      i += 2;
+    }
+  }
+  grp_sync();
+  return rc;
+}
+
+}  // namespace ce

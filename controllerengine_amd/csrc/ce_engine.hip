@@ -18,7 +18,7 @@
 
 namespace ce {
 // kernels (ce_kernels_fp64.hip, ce_kernels_mixed.hip)
-void launch_eig_fp64(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
@@ -169,7 +169,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
 
   // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
   HIPCHK(hipEventRecord(E->ev[0], st));
-  launch_eig_fp64(P, B, st);
+  launch_eig_fp64(H.ns, H.nt, P, B, st);
   launch_compact(B, B.d_y != nullptr, st);
   HIPCHK(hipEventRecord(E->ev[1], st));
   // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
