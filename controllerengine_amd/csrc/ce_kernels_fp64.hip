@@ -55,11 +55,11 @@ __global__ __launch_bounds__(64) void k_compact(StageBufs B, int all) {
 // modes with the adjacent-duplicate rule (src/modalsetup.cu:8-47), the unstable flag
 // (src/modalsetup.cu:26) and the nominal closed loop of the margins stage (src/allmargin.cu:61-79).
 //
-// G lanes per candidate (G >= the larger of nt and 2 ns), 64 / G candidates per single-wave
-// workgroup.  Lane l builds row l of each matrix and owns column / row l in the eigensolver.
-// Pass 0 solves LIN_A, pass 1 the nominal loop, through one inlined copy of the solver.
+// G lanes per candidate, 64 / G candidates per single-wave workgroup; lane l builds row l of the
+// matrix and owns column / row l in the eigensolver.  k_eig_lin (G >= nt) solves LIN_A and derives
+// modes and flags; k_eig_nom (G >= 2 ns) solves the nominal loop of the candidates that need it.
 template <int G>
-__global__ __launch_bounds__(64) void k_eig_grp(const ProblemDev* __restrict__ P, StageBufs B) {
+__global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P, StageBufs B) {
   constexpr int S = G + 1, CPW = 64 / G;
   __shared__ double smat[CPW][G * S];
   __shared__ double svec[CPW][5 * G];
@@ -71,119 +71,114 @@ __global__ __launch_bounds__(64) void k_eig_grp(const ProblemDev* __restrict__ P
   double* WR = V + G;
   double* WI = V + 2 * G;
   double* MAG = V + 3 * G;
+  double* IM = V + 4 * G;
   const int ns = P->ns, nd = P->nd, nt = P->nt, n = B.n;
   const float* g = B.gains + (size_t)c * P->cuda_w;
-  const double Ll = (l < ns) ? (double)g[ns + l] : 0.0;
-  double oar[MAXS];
-#pragma unroll
-  for (int j = 0; j < MAXS; j++) oar[j] = 0.0;
-  double* row = B.info + (size_t)c * P->info_w;
-  double* sc = row + P->info_scoring;
-  const double sc0_in = sc[0];
-  int flags = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    int m;
-    if (pass == 0) {
-      m = nt;
-      if (l < ns) {
-#pragma unroll
-        for (int j = 0; j < MAXS; j++) {
-          if (j < ns) {
-            const double Kj = g[j];
-            const double bk = P->sb[l] * Kj;
-            const double cdk = P->sc[j] - P->sd * Kj;
-            const double lcd = Ll * cdk;
-            oar[j] = (P->sa[l * ns + j] - bk) - lcd;
-            T[l * S + j] = oar[j];
-          }
-        }
-        for (int j = 0; j < nd; j++) T[l * S + ns + j] = Ll * P->dc[j];
-        double* oa = B.open_a + (size_t)c * ns * ns + (size_t)l * ns;
-#pragma unroll
-        for (int j = 0; j < MAXS; j++)
-          if (j < ns) oa[j] = oar[j];
-      } else if (l < nt) {
-        const int i = l - ns;
-        for (int j = 0; j < ns; j++) T[l * S + j] = -(P->dbu[i] * (double)g[j]);
-        for (int j = 0; j < nd; j++) T[l * S + ns + j] = P->da[i * nd + j];
-      }
-      if (B.diag && B.d_lina && l < nt) {
-        float* dl = B.d_lina + (size_t)c * nt * nt + (size_t)l * nt;
-        for (int j = 0; j < nt; j++) dl[j] = (float)T[l * S + j];
-      }
-    } else {
-      // nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-67)
-      m = 2 * ns;
-      if (l < ns) {
-#pragma unroll
-        for (int j = 0; j < MAXS; j++)
-          if (j < ns) T[l * S + j] = oar[j] - P->pd * (Ll * (double)g[j]);
-        for (int j = 0; j < ns; j++) T[l * S + ns + j] = -(Ll * P->pc[j]);
-      } else if (l < m) {
-        const int i = l - ns;
-        for (int j = 0; j < ns; j++) T[l * S + j] = P->pb[i] * (double)g[j];
-        for (int j = 0; j < ns; j++) T[l * S + ns + j] = P->pa[i * ns + j];
-      }
+  if (l < ns) {
+    const double Ll = g[ns + l];
+    double* oa = B.open_a + (size_t)c * ns * ns + (size_t)l * ns;
+    for (int j = 0; j < ns; j++) {
+      const double Kj = g[j];
+      const double bk = P->sb[l] * Kj;
+      const double cdk = P->sc[j] - P->sd * Kj;
+      const double lcd = Ll * cdk;
+      const double v = (P->sa[l * ns + j] - bk) - lcd;
+      T[l * S + j] = v;
+      oa[j] = v;
     }
-    grp_sync();
-    const int rc = eig_grp<G>(T, V, WR, WI, m, l);
-    if (pass == 0) {
-      if (rc != 0 && l < nt) { WR[l] = cedm::qnan(); WI[l] = cedm::qnan(); }
-      grp_sync();
-      double re = 0.0, im = 0.0;
-      if (l < nt) {
-        const double wr = WR[l], wi = WI[l];
-        if (B.diag && B.d_eig) {
-          float* de = B.d_eig + (size_t)c * 2 * nt;
-          de[2 * l] = (float)wr;
-          de[2 * l + 1] = (float)wi;
-        }
-        const double mag = cedm::hypot(wr, wi);
-        MAG[l] = mag;
-        re = cedm::log(cedm::fmax_(mag, EPS30)) / P->ts;
-        im = cedm::atan2(wi, wr) / P->ts;
-        V[l] = re;
-        V[4 * G + l] = im;
-      }
-      grp_sync();
-      bool keep = false;
-      if (l < nt) {
-        const double pre = (l == 0) ? BIG10 : V[l - 1];
-        const double pim = (l == 0) ? BIG10 : V[4 * G + l - 1];
-        const bool dup = cedm::fabs_(re - pre) < EPS6 && cedm::fabs_(cedm::fabs_(im) - cedm::fabs_(pim)) < EPS6;
-        keep = !dup;
-      }
-      const unsigned long long gmask = ((G == 64) ? ~0ull : ((1ull << G) - 1ull)) << (gi * G);
-      const unsigned long long kb = __ballot(keep) & gmask;
-      if (keep) {
-        const int u = __popcll(kb & ((1ull << threadIdx.x) - 1ull));
-        B.mode_re[(size_t)u * n + c] = re;
-        B.mode_im[(size_t)u * n + c] = cedm::fabs_(im);
-      }
-      double max_mag = 0.0;
-      for (int i = 0; i < nt; i++) max_mag = cedm::fmax_(max_mag, MAG[i]);
-      if (max_mag > 1.0) flags |= F_UNSTABLE;
-      if (l == 0) {
-        B.uniq[c] = __popcll(kb);
-        if (flags & F_UNSTABLE) {
-          sc[0] = P->unstable_err + cedm::log(max_mag) / P->ts;
-          sc[1] = sc[0];
-        }
-      }
-      if ((flags & F_UNSTABLE) || sc0_in >= P->unstable_err) {
-        flags |= F_NOMINAL_SKIPPED;
-        break;
-      }
-      grp_sync();
-    } else {
-      bool bad = rc != 0;
-      const bool mine = rc == 0 && l < m && cedm::hypot(WR[l], WI[l]) >= 1.0;
-      const unsigned long long gmask = ((G == 64) ? ~0ull : ((1ull << G) - 1ull)) << (gi * G);
-      if (__ballot(mine) & gmask) bad = true;
-      if (bad) flags |= F_NOMINAL_UNSTABLE;
-    }
+    for (int j = 0; j < nd; j++) T[l * S + ns + j] = Ll * P->dc[j];
+  } else if (l < nt) {
+    const int i = l - ns;
+    for (int j = 0; j < ns; j++) T[l * S + j] = -(P->dbu[i] * (double)g[j]);
+    for (int j = 0; j < nd; j++) T[l * S + ns + j] = P->da[i * nd + j];
   }
-  if (l == 0) B.flags[c] = flags;
+  if (B.diag && B.d_lina && l < nt) {
+    float* dl = B.d_lina + (size_t)c * nt * nt + (size_t)l * nt;
+    for (int j = 0; j < nt; j++) dl[j] = (float)T[l * S + j];
+  }
+  grp_sync();
+  const int rc = eig_grp<G>(T, V, WR, WI, nt, l);
+  if (rc != 0 && l < nt) { WR[l] = cedm::qnan(); WI[l] = cedm::qnan(); }
+  grp_sync();
+  double re = 0.0, im = 0.0;
+  if (l < nt) {
+    const double wr = WR[l], wi = WI[l];
+    if (B.diag && B.d_eig) {
+      float* de = B.d_eig + (size_t)c * 2 * nt;
+      de[2 * l] = (float)wr;
+      de[2 * l + 1] = (float)wi;
+    }
+    const double mag = cedm::hypot(wr, wi);
+    MAG[l] = mag;
+    re = cedm::log(cedm::fmax_(mag, EPS30)) / P->ts;
+    im = cedm::atan2(wi, wr) / P->ts;
+    V[l] = re;
+    IM[l] = im;
+  }
+  grp_sync();
+  bool keep = false;
+  if (l < nt) {
+    const double pre = (l == 0) ? BIG10 : V[l - 1];
+    const double pim = (l == 0) ? BIG10 : IM[l - 1];
+    const bool dup = cedm::fabs_(re - pre) < EPS6 && cedm::fabs_(cedm::fabs_(im) - cedm::fabs_(pim)) < EPS6;
+    keep = !dup;
+  }
+  const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
+  const unsigned long long kb = __ballot(keep) & gmask;
+  if (keep) {
+    const int u = __popcll(kb & ((1ull << threadIdx.x) - 1ull));
+    B.mode_re[(size_t)u * n + c] = re;
+    B.mode_im[(size_t)u * n + c] = cedm::fabs_(im);
+  }
+  if (l == 0) {
+    double max_mag = 0.0;
+    for (int i = 0; i < nt; i++) max_mag = cedm::fmax_(max_mag, MAG[i]);
+    double* sc = B.info + (size_t)c * P->info_w + P->info_scoring;
+    int flags = 0;
+    if (max_mag > 1.0) {
+      sc[0] = P->unstable_err + cedm::log(max_mag) / P->ts;
+      sc[1] = sc[0];
+      flags |= F_UNSTABLE;
+    }
+    if ((flags & F_UNSTABLE) || sc[0] >= P->unstable_err) flags |= F_NOMINAL_SKIPPED;
+    B.uniq[c] = __popcll(kb);
+    B.flags[c] = flags;
+  }
+}
+
+// nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79)
+template <int G>
+__global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P, StageBufs B) {
+  constexpr int S = G + 1, CPW = 64 / G;
+  __shared__ double smat[CPW][G * S];
+  __shared__ double svec[CPW][3 * G];
+  const int gi = threadIdx.x / G, l = threadIdx.x % G;
+  const int c = blockIdx.x * CPW + gi;
+  if (c >= B.n) return;
+  const int flags = B.flags[c];
+  if (flags & F_NOMINAL_SKIPPED) return;
+  double* T = smat[gi];
+  double* V = svec[gi];
+  double* WR = V + G;
+  double* WI = V + 2 * G;
+  const int ns = P->ns, m = 2 * ns;
+  const float* g = B.gains + (size_t)c * P->cuda_w;
+  if (l < ns) {
+    const double Ll = g[ns + l];
+    const double* oa = B.open_a + (size_t)c * ns * ns + (size_t)l * ns;
+    for (int j = 0; j < ns; j++) T[l * S + j] = oa[j] - P->pd * (Ll * (double)g[j]);
+    for (int j = 0; j < ns; j++) T[l * S + ns + j] = -(Ll * P->pc[j]);
+  } else if (l < m) {
+    const int i = l - ns;
+    for (int j = 0; j < ns; j++) T[l * S + j] = P->pb[i] * (double)g[j];
+    for (int j = 0; j < ns; j++) T[l * S + ns + j] = P->pa[i * ns + j];
+  }
+  grp_sync();
+  const int rc = eig_grp<G>(T, V, WR, WI, m, l);
+  const bool out = rc == 0 && l < m && cedm::hypot(WR[l], WI[l]) >= 1.0;
+  const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
+  const bool bad = rc != 0 || (__ballot(out) & gmask) != 0ull;
+  if (l == 0 && bad) B.flags[c] = flags | F_NOMINAL_UNSTABLE;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -482,10 +477,13 @@ bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hi
 }
 
 void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  const int need = (nt > 2 * ns) ? nt : 2 * ns;
-  if (need <= 4) hipLaunchKernelGGL(k_eig_grp<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
-  else if (need <= 8) hipLaunchKernelGGL(k_eig_grp<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
-  else hipLaunchKernelGGL(k_eig_grp<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  if (nt <= 4) hipLaunchKernelGGL(k_eig_lin<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
+  else if (nt <= 8) hipLaunchKernelGGL(k_eig_lin<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
+  else hipLaunchKernelGGL(k_eig_lin<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  const int m = 2 * ns;
+  if (m <= 4) hipLaunchKernelGGL(k_eig_nom<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
+  else if (m <= 8) hipLaunchKernelGGL(k_eig_nom<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
+  else hipLaunchKernelGGL(k_eig_nom<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
 }
 
 void launch_compact(const StageBufs& B, int all, hipStream_t st) {
