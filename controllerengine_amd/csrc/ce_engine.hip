@@ -58,7 +58,7 @@ struct ce_engine {
     size_t cap = 0;
   };
   Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
-  Buf mr32, refined, rlist, rcount, rslots, rcnt;
+  Buf mr32, refined, rlist, rcount, rslots, rcnt, atbp;
   float* diag_y = nullptr;
   float* diag_lina = nullptr;
   float* diag_eig = nullptr;
@@ -149,12 +149,23 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     B.rcount = (int32_t*)ensure(E->rcount, sizeof(int32_t));
     B.rslots = (int32_t*)ensure(E->rslots, nn * RMAX * sizeof(int32_t));
     B.rcnt = (int32_t*)ensure(E->rcnt, nn * sizeof(int32_t));
+    // time parts of the right-hand-side pass: enough waves to fill the chip, >= 64 samples each
+    const int mb = (nt + 7) / 8;
+    const long long waves = (long long)mb * (long long)((nn + 63) / 64);
+    int parts = (int)std::max(1LL, std::min(16LL, 8192LL / std::max(1LL, waves)));
+    parts = std::max(1, std::min(parts, H.fit_count / 64));
+    const int chunk = (H.fit_count + parts - 1) / parts;
+    B.atb_chunk = std::max(1, chunk);
+    B.atb_parts = (H.fit_count + B.atb_chunk - 1) / B.atb_chunk;
+    if (B.atb_parts < 1) B.atb_parts = 1;
+    B.atbp = (double2*)ensure(E->atbp, (size_t)B.atb_parts * nt * nn * sizeof(double2));
   }
   B.diag = (E->diag_y || E->diag_lina || E->diag_eig || E->diag_ata || E->diag_atb) ? 1 : 0;
   B.d_y = E->diag_y; B.d_lina = E->diag_lina; B.d_eig = E->diag_eig; B.d_ata = E->diag_ata; B.d_atb = E->diag_atb;
   if (B.diag) {
-    if (B.d_ata) HIPCHK(hipMemsetAsync(B.d_ata, 0, nn * 4 * nt * nt * sizeof(float), st));
-    if (B.d_atb) HIPCHK(hipMemsetAsync(B.d_atb, 0, nn * 2 * nt * sizeof(float), st));
+    // zero fills by kernel (no memset nodes in code a caller may capture into a graph)
+    if (B.d_ata) launch_fill_i32((int32_t*)B.d_ata, 0, nn * 4 * nt * nt, st);
+    if (B.d_atb) launch_fill_i32((int32_t*)B.d_atb, 0, nn * 2 * nt, st);
   }
   launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
   launch_fill_i32(B.sweep_fail, 0, nn, st);
@@ -231,7 +242,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
                   &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
-                  &E->rcount, &E->rslots, &E->rcnt})
+                  &E->rcount, &E->rslots, &E->rcnt, &E->atbp})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

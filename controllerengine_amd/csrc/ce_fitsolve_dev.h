@@ -1,8 +1,7 @@
 // Normal-equation solve and behaviour score of the modal fit, reference src/modalsolve.cu:150-203:
 // column scaling by sqrt(diag), LDL^T with symmetric pivoting (Eigen LDLT<Lower> restated; zero
 // pivots solve to zero), per-mode energy on the unscaled Gram, energy-weighted wn and zeta of the
-// oscillatory modes, hinge errors.  Shared by the FP64 fit and the mixed closed-form fit; runs on
-// one lane per candidate over LDS-resident data.
+// oscillatory modes, hinge errors.  Shared by the FP64 fit and the mixed closed-form fit.
 #pragma once
 
 #include <float.h>
@@ -13,22 +12,33 @@ namespace ce {
 
 __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 
+// Group-parallel form: G lanes of one candidate (l = lane in group) work on LDS-resident data.
+// Element-wise steps are split over lanes by their independent index; every element sees the
+// same operations in the same order as in a sequential solve (the pivot search and the
+// triangular substitutions run on every lane / on lane 0 in the original order), so the result
+// does not depend on G.  The lanes of the group must be convergent; phases are separated by a
+// workgroup-scope fence (single-wave workgroups).
 // G2: packed lower triangle of the 2nt x 2nt Gram (overwritten), atb: right-hand side; dg, sd, cs,
-// x: scratch vectors of 2nt.  u: unique modes; modes read from B.mode_re/mode_im of candidate c.
-__device__ __noinline__ void solve_and_score(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
-                                             double* row, double* G2, const double* atb, double* dg, double* sd,
-                                             double* cs, double* x) {
-  const int nt = P->nt, nc2 = 2 * nt;
+// x, tmp: LDS vectors of 2nt; trans: LDS ints of 2nt.  u: unique modes of candidate c.
+__device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+template <int G>
+__device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
+                                                    double* row, double* G2, const double* atb, double* dg,
+                                                    double* sd, double* cs, double* x, double* tmp, int* trans,
+                                                    int l) {
+  const int nt = P->nt, N2 = 2 * nt;
   const int n = B.n;
   double* sc = row + P->info_scoring;
-  for (int i = 0; i < nc2; i++) {
+  for (int i = l; i < N2; i += G) {
     dg[i] = G2[tri(i, i)];
-    sd[i] = (i + 1 < nc2) ? G2[tri(i + 1, i)] : 0.0;
+    sd[i] = (i + 1 < N2) ? G2[tri(i + 1, i)] : 0.0;
     const double s = cedm::sqrt_(dg[i]);
     cs[i] = (s > EPS6) ? s : 1.0;
   }
+  fs_sync();
   // scaled system D^-1 G D^-1 (lower triangle) and right-hand side D^-1 b
-  for (int i = 0; i < nc2; i++) {
+  for (int i = l; i < N2; i += G) {
     const double inv_i = 1.0 / cs[i];
     for (int j = 0; j <= i; j++) {
       const double inv_j = 1.0 / cs[j];
@@ -36,10 +46,9 @@ __device__ __noinline__ void solve_and_score(const ProblemDev* __restrict__ P, c
     }
     x[i] = atb[i] / cs[i];
   }
+  fs_sync();
   // LDL^T with symmetric pivoting on the largest remaining diagonal (lower storage)
-  int trans[MAXNC];
   bool all_zero = false;
-  const int N2 = nc2;
   for (int k = 0; k < N2; k++) {
     int idx = k;
     double best = cedm::fabs_(G2[tri(k, k)]);
@@ -47,71 +56,106 @@ __device__ __noinline__ void solve_and_score(const ProblemDev* __restrict__ P, c
       const double a = cedm::fabs_(G2[tri(i, i)]);
       if (a > best) { best = a; idx = i; }
     }
-    trans[k] = idx;
+    fs_sync();
+    if (l == 0) trans[k] = idx;
     if (k != idx) {
-      for (int j = 0; j < k; j++) { const double t = G2[tri(k, j)]; G2[tri(k, j)] = G2[tri(idx, j)]; G2[tri(idx, j)] = t; }
-      for (int i = idx + 1; i < N2; i++) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(i, idx)]; G2[tri(i, idx)] = t; }
-      { const double t = G2[tri(k, k)]; G2[tri(k, k)] = G2[tri(idx, idx)]; G2[tri(idx, idx)] = t; }
-      for (int i = k + 1; i < idx; i++) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(idx, i)]; G2[tri(idx, i)] = t; }
+      for (int j = l; j < k; j += G) { const double t = G2[tri(k, j)]; G2[tri(k, j)] = G2[tri(idx, j)]; G2[tri(idx, j)] = t; }
+      for (int i = idx + 1 + l; i < N2; i += G) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(i, idx)]; G2[tri(i, idx)] = t; }
+      if (l == 0) { const double t = G2[tri(k, k)]; G2[tri(k, k)] = G2[tri(idx, idx)]; G2[tri(idx, idx)] = t; }
+      for (int i = k + 1 + l; i < idx; i += G) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(idx, i)]; G2[tri(idx, i)] = t; }
+      fs_sync();
     }
     if (k > 0) {
-      double tmp[MAXNC];
-      for (int j = 0; j < k; j++) tmp[j] = G2[tri(j, j)] * G2[tri(k, j)];
-      double acc = 0.0;
-      for (int j = 0; j < k; j++) acc = cedm::fma_(G2[tri(k, j)], tmp[j], acc);
-      G2[tri(k, k)] = G2[tri(k, k)] - acc;
-      for (int i = k + 1; i < N2; i++) {
+      for (int j = l; j < k; j += G) tmp[j] = G2[tri(j, j)] * G2[tri(k, j)];
+      fs_sync();
+      if (l == 0) {
+        double acc = 0.0;
+        for (int j = 0; j < k; j++) acc = cedm::fma_(G2[tri(k, j)], tmp[j], acc);
+        G2[tri(k, k)] = G2[tri(k, k)] - acc;
+      }
+      for (int i = k + 1 + l; i < N2; i += G) {
         double a = 0.0;
         for (int j = 0; j < k; j++) a = cedm::fma_(G2[tri(i, j)], tmp[j], a);
         G2[tri(i, k)] = G2[tri(i, k)] - a;
       }
+      fs_sync();
     }
     const double akk = G2[tri(k, k)];
     const bool valid = cedm::fabs_(akk) > 0.0;
     if (k == 0 && !valid) { all_zero = true; break; }
     if (valid)
-      for (int i = k + 1; i < N2; i++) G2[tri(i, k)] = G2[tri(i, k)] / akk;
+      for (int i = k + 1 + l; i < N2; i += G) G2[tri(i, k)] = G2[tri(i, k)] / akk;
+    fs_sync();
   }
-  if (all_zero) {
-    for (int i = 0; i < N2; i++) x[i] = 0.0;
-  } else {
-    for (int k = 0; k < N2; k++) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
-    for (int i = 0; i < N2; i++) {
-      double acc = 0.0;
-      for (int j = 0; j < i; j++) acc = cedm::fma_(G2[tri(i, j)], x[j], acc);
-      x[i] = x[i] - acc;
+  fs_sync();
+  if (l == 0) {
+    if (all_zero) {
+      for (int i = 0; i < N2; i++) x[i] = 0.0;
+    } else {
+      for (int k = 0; k < N2; k++) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
+      for (int i = 0; i < N2; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < i; j++) acc = cedm::fma_(G2[tri(i, j)], x[j], acc);
+        x[i] = x[i] - acc;
+      }
+      for (int i = 0; i < N2; i++) {
+        const double d = G2[tri(i, i)];
+        x[i] = (cedm::fabs_(d) > DBL_MIN) ? x[i] / d : 0.0;
+      }
+      for (int i = N2 - 1; i >= 0; i--) {
+        double acc = 0.0;
+        for (int j = i + 1; j < N2; j++) acc = cedm::fma_(G2[tri(j, i)], x[j], acc);
+        x[i] = x[i] - acc;
+      }
+      for (int k = N2 - 1; k >= 0; k--) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
     }
-    for (int i = 0; i < N2; i++) {
-      const double d = G2[tri(i, i)];
-      x[i] = (cedm::fabs_(d) > DBL_MIN) ? x[i] / d : 0.0;
-    }
-    for (int i = N2 - 1; i >= 0; i--) {
-      double acc = 0.0;
-      for (int j = i + 1; j < N2; j++) acc = cedm::fma_(G2[tri(j, i)], x[j], acc);
-      x[i] = x[i] - acc;
-    }
-    for (int k = N2 - 1; k >= 0; k--) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
+    for (int i = 0; i < N2; i++) x[i] = x[i] / cs[i];
   }
-  for (int i = 0; i < N2; i++) x[i] = x[i] / cs[i];
+  fs_sync();
 
-  // per-mode energy, energy-weighted wn and zeta of the oscillatory modes (src/modalsolve.cu:168-190)
+  // per-mode energy, energy-weighted wn and zeta of the oscillatory modes (src/modalsolve.cu:168-190);
+  // per-mode terms on lane e, the sums over modes in mode order on lane 0
   double* md = row + P->info_modal;
   double* fr = row + P->info_freqresp;
-  double esum = 0.0, lw = 0.0, zw = 0.0;
-  for (int e = 0; e < u; e++) {
+  static_assert(G >= MAXT, "one lane per mode");
+  double* t_en = tmp;           // energy
+  double* t_lw = dg;            // energy * log(wn_e)
+  double* t_zw = sd;            // energy * zeta_e
+  int* t_osc = trans;           // oscillatory mode: counted in the sums
+  const int e = l;
+  double energy = 0.0, lwe = 0.0, zwe = 0.0;
+  bool osc = false;
+  if (e < u) {
     const int id = 2 * e;
     const double c0 = x[id], c1 = x[id + 1];
     const double g00 = dg[id], g01 = sd[id], g11 = dg[id + 1];
-    const double energy = ((c0 * c0) * g00 + ((2.0 * c0) * c1) * g01) + (c1 * c1) * g11;
+    energy = ((c0 * c0) * g00 + ((2.0 * c0) * c1) * g01) + (c1 * c1) * g11;
     const double re_e = B.mode_re[(size_t)e * n + c], im_e = B.mode_im[(size_t)e * n + c];
     const double wn_e = cedm::fmax_(cedm::sqrt_(cedm::fma_(re_e, re_e, im_e * im_e)), EPS6);
     md[id] = wn_e;
     md[id + 1] = -re_e / wn_e;
     md[2 * nt + e] = energy;
-    if (im_e < EPS6) continue;
-    esum = esum + energy;
-    lw = lw + energy * cedm::log(wn_e);
-    zw = zw + energy * (-re_e / wn_e);
+    osc = !(im_e < EPS6);
+    if (osc) {
+      lwe = energy * cedm::log(wn_e);
+      zwe = energy * (-re_e / wn_e);
+    }
+  }
+  fs_sync();
+  if (e < u) {
+    t_en[e] = energy;
+    t_lw[e] = lwe;
+    t_zw[e] = zwe;
+    t_osc[e] = osc ? 1 : 0;
+  }
+  fs_sync();
+  if (l != 0) return;
+  double esum = 0.0, lw = 0.0, zw = 0.0;
+  for (int q = 0; q < u; q++) {
+    if (!t_osc[q]) continue;
+    esum = esum + t_en[q];
+    lw = lw + t_lw[q];
+    zw = zw + t_zw[q];
   }
   md[3 * nt] = (double)u;
   double wn, zeta;

@@ -206,7 +206,8 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   constexpr int G = S::G, CPW = S::CPW, NCP = S::NCP, NM = S::NM, TB = S::TB;
   __shared__ double phi[CPW][TB][NCP];
   __shared__ double gram[CPW][S::NTRI];
-  __shared__ double vec[CPW][5][NCP];  // atb, diag, subdiag, col scale, solution
+  __shared__ double vec[CPW][6][NCP];  // atb, diag, subdiag, col scale, solution, scratch
+  __shared__ int itr[CPW][NCP];
   __shared__ double modes[CPW][S::NM][3];
 
   const int lane = threadIdx.x;
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
     }
   }
   __syncthreads();
-  if (!active || g != 0) return;
+  if (!active) return;
 
   double* G2 = gram[slot];
   double* atb = vec[slot][0];
@@ -355,18 +356,18 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   double* sd = vec[slot][2];
   double* cs = vec[slot][3];
   double* x = vec[slot][4];
-  if (B.diag && B.d_ata) {
+  if (g == 0 && B.diag && B.d_ata) {
     float* da = B.d_ata + (size_t)c * nc2 * nc2;
     for (int i = 0; i < nc2; i++)
       for (int j = 0; j < nc2; j++) da[i * nc2 + j] = (j <= i) ? (float)G2[tri(i, j)] : 0.0f;
   }
-  if (B.diag && B.d_atb) {
+  if (g == 0 && B.diag && B.d_atb) {
     float* db = B.d_atb + (size_t)c * nc2;
     for (int i = 0; i < nc2; i++) db[i] = (float)atb[i];
   }
   double* sc = row + P->info_scoring;
   if (sc[0] != 0.0) return;  // modalsolve skips flagged candidates (src/modalsolve.cu:112)
-  solve_and_score(P, B, c, u, row, G2, atb, dg, sd, cs, x);
+  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g);
 }
 
 // ------------------------------------------------------------------------------------------------

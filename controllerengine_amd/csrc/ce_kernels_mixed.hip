@@ -64,6 +64,60 @@ __device__ __forceinline__ Cplx geo_sum(double sig, double beta, double tau0, do
   return cmul(Cplx{mag * c, mag * s}, ratio);
 }
 
+// Right-hand side of the fit, sum_t phi_e(t) (y(t) - yss), by Horner's rule in double:
+// sum_{t in part} (y_t - yss) z_e^(t - t_part) with z_e = exp(s_e ts), over the part's samples in
+// reverse.  Lane = list slot (coalesced reads of the SoA trajectory), 8 modes per lane
+// (blockIdx.y selects modes 8y..8y+7), blockIdx.z = time part.
+constexpr int ATB_MODES = 8;
+__global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= *B.count) return;
+  const int c = B.list[idx];
+  const double* row = B.info + (size_t)c * P->info_w;
+  if (B.flags[c] & F_UNSTABLE) return;
+  if (!(row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)))) return;
+  const int u = B.uniq[c];
+  const int e0 = blockIdx.y * ATB_MODES;
+  if (e0 >= u) return;
+  const int n = B.n, N = P->fit_count;
+  const int tb = blockIdx.z * B.atb_chunk;
+  const int te = (tb + B.atb_chunk < N) ? tb + B.atb_chunk : N;
+  const double ts = P->ts;
+  const double yss = row[P->info_freqresp + 3];
+  double zr[ATB_MODES], zi[ATB_MODES], ar[ATB_MODES], ai[ATB_MODES];
+#pragma unroll
+  for (int k = 0; k < ATB_MODES; k++) {
+    zr[k] = 0.0;
+    zi[k] = 0.0;
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+    if (e0 + k < u) {
+      const double re = B.mode_re[(size_t)(e0 + k) * n + c], im = B.mode_im[(size_t)(e0 + k) * n + c];
+      const double ms = cedm::exp(re * ts);
+      double s, co;
+      cedm::sincos(im * ts, s, co);
+      zr[k] = ms * co;
+      zi[k] = ms * s;
+    }
+  }
+  const float* ycol = B.ybuf32 + idx;
+#pragma unroll 4
+  for (int t = te - 1; t >= tb; t--) {
+    const double y = (double)ycol[(size_t)t * n] - yss;
+#pragma unroll
+    for (int k = 0; k < ATB_MODES; k++) {
+      const double nr = __builtin_fma(ar[k], zr[k], __builtin_fma(-ai[k], zi[k], y));
+      const double ni = __builtin_fma(ar[k], zi[k], ai[k] * zr[k]);
+      ar[k] = nr;
+      ai[k] = ni;
+    }
+  }
+  double2* out = B.atbp + ((size_t)blockIdx.z * P->nt + e0) * n + idx;
+#pragma unroll
+  for (int k = 0; k < ATB_MODES; k++)
+    if (e0 + k < u) out[(size_t)k * n] = make_double2(ar[k], ai[k]);
+}
+
 template <int NB>
 struct CfShape {
   static constexpr int G = 16;
@@ -72,13 +126,15 @@ struct CfShape {
   static constexpr int NTRI = NCP * (NCP + 1) / 2;
 };
 
+// Gram matrix in closed form, right-hand side from the partial sums of k_atb, then the LDL^T
+// solve and score.  G = 16 lanes per candidate, 4 candidates per single-wave workgroup.
 template <int NB>
 __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, StageBufs B) {
   using S = CfShape<NB>;
   constexpr int G = S::G, CPW = S::CPW, NTRI = S::NTRI, NCP = S::NCP;
   __shared__ double gram[CPW][NTRI];
-  __shared__ double vec[CPW][5][NCP];
-  __shared__ double part_sum[CPW][G][4];
+  __shared__ double vec[CPW][6][NCP];
+  __shared__ int itr[CPW][NCP];
   const int lane = threadIdx.x;
   const int slot = lane / G, g = lane % G;
   const int idx = blockIdx.x * CPW + slot;
@@ -101,43 +157,25 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
   const int f0 = P->fit_start_id, N = P->fit_count;
   const double tau0 = (double)f0 * ts;
   if (active) {
-    // right-hand side sum_t phi(t)(y(t) - yss): lanes split as (mode, time part) so that all G
-    // lanes work when u < G; each part restarts the basis at its first sample
-    int sp = 1;
-    while (sp * 2 * (u > 0 ? u : 1) <= G) sp *= 2;
-    const int e = g / sp, part = g % sp;
-    double ayr = 0.0, ayi = 0.0, asr = 0.0, asi = 0.0;
-    if (e < u) {
+    if (g < u) {
+      // Atb_e = sum_parts phi_e(t_part) * partial_e,part (complex)
+      const int e = g;
       const double re = B.mode_re[(size_t)e * n + c], im = B.mode_im[(size_t)e * n + c];
       const double sh = cedm::fmax_(re * t0, re * t1);
-      const int chunk = (N + sp - 1) / sp;
-      const int tb = part * chunk, te = (tb + chunk < N) ? tb + chunk : N;
-      const double tau = (double)(f0 + tb) * ts;
-      const double m0 = cedm::exp(re * tau - sh);
-      double s0, c0;
-      cedm::sincos(im * tau, s0, c0);
-      double br = m0 * c0, bi = m0 * s0;
-      const double ms = cedm::exp(re * ts);
-      double s1, c1;
-      cedm::sincos(im * ts, s1, c1);
-      const double zr = ms * c1, zi = ms * s1;
-      const float* ycol = B.ybuf32 + idx;
-#pragma unroll 8
-      for (int t = tb; t < te; t++) {
-        const double y = (double)ycol[(size_t)t * n];
-        ayr = __builtin_fma(br, y, ayr);
-        ayi = __builtin_fma(bi, y, ayi);
-        asr = asr + br;
-        asi = asi + bi;
-        const double nr = br * zr - bi * zi;
-        bi = br * zi + bi * zr;
-        br = nr;
+      double sr = 0.0, si = 0.0;
+      for (int q = 0; q < B.atb_parts; q++) {
+        const double tau = (double)(f0 + q * B.atb_chunk) * ts;
+        const double m0 = cedm::exp(re * tau - sh);
+        double s0, c0;
+        cedm::sincos(im * tau, s0, c0);
+        const double br = m0 * c0, bi = m0 * s0;
+        const double2 pv = B.atbp[((size_t)q * nt + e) * n + idx];
+        sr = sr + (br * pv.x - bi * pv.y);
+        si = si + (br * pv.y + bi * pv.x);
       }
+      vec[slot][0][2 * e] = sr;
+      vec[slot][0][2 * e + 1] = si;
     }
-    part_sum[slot][g][0] = ayr;
-    part_sum[slot][g][1] = ayi;
-    part_sum[slot][g][2] = asr;
-    part_sum[slot][g][3] = asi;
     // Gram in closed form: mode pairs (e >= f) spread over the group's lanes
     const int npairs = u * (u + 1) / 2;
     for (int p = g; p < npairs; p += G) {
@@ -158,45 +196,21 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
     }
   }
   __syncthreads();
-  if (active && g < G / 2) {
-    int sp = 1;
-    while (sp * 2 * (u > 0 ? u : 1) <= G) sp *= 2;
-    const int e = g;
-    if (e < u) {
-      double ayr = 0.0, ayi = 0.0, asr = 0.0, asi = 0.0;
-      for (int q = 0; q < sp; q++) {
-        ayr = ayr + part_sum[slot][e * sp + q][0];
-        ayi = ayi + part_sum[slot][e * sp + q][1];
-        asr = asr + part_sum[slot][e * sp + q][2];
-        asi = asi + part_sum[slot][e * sp + q][3];
-      }
-      const double yss = row[P->info_freqresp + 3];
-      vec[slot][0][2 * e] = ayr - yss * asr;
-      vec[slot][0][2 * e + 1] = ayi - yss * asi;
-    }
-  }
-  if (active && g >= G / 2 && g < u) {
-    // modes G/2.. (u > G/2 means one time part per mode)
-    const int e = g;
-    const double yss = row[P->info_freqresp + 3];
-    vec[slot][0][2 * e] = part_sum[slot][e][0] - yss * part_sum[slot][e][2];
-    vec[slot][0][2 * e + 1] = part_sum[slot][e][1] - yss * part_sum[slot][e][3];
-  }
-  __syncthreads();
-  if (!active || g != 0) return;
+  if (!active) return;
   double* G2 = gram[slot];
   double* atb = vec[slot][0];
-  if (B.diag && B.d_ata) {
+  if (g == 0 && B.diag && B.d_ata) {
     float* da = B.d_ata + (size_t)c * nc2 * nc2;
     for (int i = 0; i < nc2; i++)
       for (int j = 0; j < nc2; j++) da[i * nc2 + j] = (j <= i) ? (float)G2[tri(i, j)] : 0.0f;
   }
-  if (B.diag && B.d_atb) {
+  if (g == 0 && B.diag && B.d_atb) {
     float* db = B.d_atb + (size_t)c * nc2;
     for (int i = 0; i < nc2; i++) db[i] = (float)atb[i];
   }
   if (row[P->info_scoring] != 0.0) return;
-  solve_and_score(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4]);
+  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4],
+                         vec[slot][5], itr[slot], g);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -413,6 +427,7 @@ bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, h
 
 bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int nb = (2 * nt + 7) / 8;
+  hipLaunchKernelGGL(k_atb, dim3((B.n + 63) / 64, (nt + ATB_MODES - 1) / ATB_MODES, B.atb_parts), dim3(64), 0, st, P, B);
   const dim3 grid((B.n + 3) / 4);
   switch (nb) {
     case 1: hipLaunchKernelGGL(k_fitcf<1>, grid, dim3(64), 0, st, P, B); return true;

@@ -38,6 +38,10 @@ struct StageBufs {
   int32_t* rcnt;                // n: used slots per candidate
   int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
   int32_t* rcount;              // 1
+  // mixed fit: partial right-hand sides sum_{t in part} (y_t - yss) z_e^(t - t_part), complex,
+  // layout [part][mode][list slot]
+  double2* atbp;
+  int32_t atb_parts, atb_chunk;
   // diagnostics in the reference dump layout (float, candidate-major), NULL when not requested
   float* d_y;
   float* d_lina;
