@@ -23,31 +23,41 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_flops(problem, info64, d):
-    """Algorithmic flops per stage for the candidates that reached each stage (formulas of the
-    measurement plan: FMA = 2, add/mul/div = 1, each non-SKIP instruction = 1)."""
+F32_KERNELS = ("ce_sim_f32", "k_sim_f32", "k_sweep32")
+
+
+def kernel_flops(problem, info64, d):
+    """Algorithmic flops per launch of each stage kernel, counted for the candidates that reach it
+    (FMA = 2, add/mul/div = 1, each non-SKIP instruction = 1; transcendentals not counted)."""
     import numpy as np
     c = problem.config
     ns, nd, nw = int(c["statespace_dim"]), int(c["discrete_dim"]), int(c["nonlin_dim"])
     T, F = int(c["time"]), int(c["freq"])
     nt = ns + nd
     ninstr = int(c["dc_instructions"]) + int(c["u_instructions"]) + sum(c["nl_instructions"] or [0])
-    sc0 = info64[:, d.info_scoring]
-    unstable = (info64[:, d.info_scoring + 1] >= float(c["unstable_err"])) & (info64[:, d.info_allmargin + 3] == 0)
-    uniq = info64[:, d.info_modal + 3 * nt]
     n = info64.shape[0]
-    per_step = 4 * ns + 5 + ns * (2 * ns + 4) + nd * (2 * nd + 2 * nw + 2) + ninstr
-    sim = float(per_step) * T * int((~unstable).sum())
-    eig = float(10 * nt ** 3 + 4 * ns * ns + 2 * ns * nd + 2 * ns) * n
-    fitted = uniq > 0
+    sc = info64[:, d.info_scoring:d.info_scoring + 3]
+    stable_lin = ~((sc[:, 0] >= float(c["unstable_err"])) & (info64[:, d.info_allmargin + 3] == 0))
+    n_sim = int(stable_lin.sum())
+    uniq = info64[:, d.info_modal + 3 * nt]
+    fitted = (uniq > 0) & stable_lin
     u = uniq[fitted]
     nc = 2 * u
     tfit = d.fit_count
-    fit = float(np.sum(tfit * (nc * (nc + 1) + 2 * nc + 4 * u + 1) + 2 * T))
-    solve = float(np.sum((2 * nt) ** 3 / 3 + 4 * (2 * nt) ** 2)) * int(fitted.sum()) / max(1, int(fitted.sum()))
-    swept = info64[:, d.info_allmargin + 3] == 1
-    sweep = float(F * (8.0 / 3.0 * ns ** 3 + 8 * ns * ns + 8 * ns + 30) + 10 * (2 * ns) ** 3) * int(swept.sum())
-    return dict(eig=eig, sim=sim, fit=fit + solve, sweep=sweep)
+    per_step = 4 * ns + 5 + ns * (2 * ns + 4) + nd * (2 * nd + 2 * nw + 2) + ninstr
+    sim = float(per_step) * T * n_sim
+    solve = float((2 * nt) ** 3 / 3 + 4 * (2 * nt) ** 2) * int(fitted.sum())
+    per_point = 8.0 / 3.0 * ns ** 3 + 8 * ns * ns + 8 * ns + 30
+    swept = int((info64[:, d.info_allmargin + 3] == 1).sum())
+    return {
+        "ce_sim_f32": sim, "ce_sim_f64": sim, "k_sim_f32": sim, "k_sim_fp64": sim,
+        "k_eig_lin": float(10 * nt ** 3 + 4 * ns * ns + 2 * ns * nd + 2 * ns) * n,
+        "k_eig_nom": float(10 * (2 * ns) ** 3) * n_sim,
+        "k_atb": float(np.sum(8.0 * u * tfit)),
+        "k_fitcf": solve + float(np.sum(u * (u + 1) / 2)) * 2 * 40,
+        "k_fit_fp64": float(np.sum(tfit * (nc * (nc + 1) + 2 * nc + 4 * u + 1) + 2 * T)) + solve,
+        "k_sweep32": F * per_point * swept, "k_sweep_fp64": F * per_point * swept,
+    }
 
 
 def cpu_baseline(problem, target_s=12.0, max_n=None):
@@ -71,11 +81,18 @@ def cpu_baseline(problem, target_s=12.0, max_n=None):
     if max_n:
         n_final = min(n_final, max_n)
     sub = problem.subset(np.arange(n_final))
+    # repeat the sample until about target_s of CPU work when the whole workload is quicker
+    reps = 0
     t0 = time.perf_counter()
-    oracle.evaluate(sub, threads=threads)
-    t = time.perf_counter() - t0
-    return dict(value=n_final / t, unit="candidate-evals/s", cores=threads, kind="port",
-                sample="first %d of %d candidates, FP64 oracle, %.1f s" % (n_final, n_total, t))
+    while True:
+        oracle.evaluate(sub, threads=threads)
+        reps += 1
+        t = time.perf_counter() - t0
+        if t >= 0.8 * target_s or n_final < n_total:
+            break
+    return dict(value=reps * n_final / t, unit="candidate-evals/s", cores=threads, kind="port",
+                sample="%d x first %d of %d candidates, FP64 oracle (all stages), %.1f s" % (
+                    reps, n_final, n_total, t))
 
 
 def main():
@@ -86,6 +103,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--n", type=int, default=0, help="candidates (default: the configuration's count)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
+    ap.add_argument("--prof-steps", type=int, default=3, help="extra steps with per-kernel events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -102,7 +120,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the candidate chain has no cross-GPU step in this build "
-              "(multi-GPU sharding is by candidate range, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(multi-GPU sharding would split candidate ranges); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -142,26 +160,31 @@ def main():
         for _ in range(args.warmup):
             eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
         torch.cuda.synchronize()
-        stage_acc = None
         t0 = time.perf_counter()
         for _ in range(args.steps):
             eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
-            st = eng.stage_times()  # synchronises on the step's last event
-            stage_acc = st if stage_acc is None else [a + b for a, b in zip(stage_acc, st)]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         info64 = i64.cpu().numpy()
-        stages = [s / args.steps for s in stage_acc]
-        results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, stages=stages, info64=info64)
+        # per-kernel durations from HIP events recorded on the launch stream, separate passes
+        eng.set_profiling(True)
+        acc = {}
+        for _ in range(args.prof_steps):
+            eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
+            for k, ms in eng.kernel_times():
+                acc.setdefault(k, []).append(ms)
+        eng.set_profiling(False)
+        kern = {k: sum(v) / len(v) for k, v in acc.items()}
+        results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, kernels=kern, info64=info64)
 
     head = modes[0][0]
     r = results[head]
-    fl = stage_flops(problem, r["info64"], d)
-    names = ["eig", "sim", "fit", "sweep", "final"]
-    stage_ms = dict(zip(names, r["stages"]))
-    dom = max(["eig", "sim", "fit", "sweep"], key=lambda k: stage_ms[k])
-    achieved = fl[dom] / (stage_ms[dom] * 1e-3) / 1e12
-    peak = FP64_PEAK_TFLOPS
+    fl = kernel_flops(problem, r["info64"], d)
+    kern = {k: v for k, v in r["kernels"].items() if k != "start"}
+    dom = max(kern, key=lambda k: kern[k]) if kern else None
+    dom_ms = kern.get(dom)
+    peak = FP32_PEAK_TFLOPS if dom in F32_KERNELS else FP64_PEAK_TFLOPS
+    achieved = fl[dom] / (dom_ms * 1e-3) / 1e12 if dom in fl else None
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (dom, args.config, head))
     if os.path.exists(tf):
@@ -184,10 +207,10 @@ def main():
             args.config, {"c1": "mass-spring-damper 2-state", "c2": "two-mass drive 4-state, 2 NL",
                           "c3": "random stable 8-state, 4 NL"}[args.config], n, cfg.time, cfg.freq, head),
                    "precision": head},
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "flops_per_launch": fl[dom]},
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+        "roofline": {"bound": "valu", "kernel": dom, "launch_ms": dom_ms, "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": (achieved / peak) if achieved is not None else None,
+                     "traffic": traffic, "flops_per_launch": fl.get(dom)},
     }
     if "fp64" in results and head != "fp64":
         line["fp64_value"] = results["fp64"]["value"]

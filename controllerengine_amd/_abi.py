@@ -68,7 +68,7 @@ class CeDerived(ctypes.Structure):
 EXPORTS = [
     "RunControllerEngine", "ce_config_parse", "ce_config_derive", "ce_plant_load_dir", "ce_engine_create",
     "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times",
-    "ce_engine_last_sim_jit", "ce_detmath_host",
+    "ce_engine_last_sim_jit", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_detmath_host",
     "ce_detmath_device",
 ]
 
@@ -99,6 +99,10 @@ def _declare(lib):
     lib.ce_engine_stage_times.restype = ctypes.c_int
     lib.ce_engine_last_sim_jit.argtypes = [vp]
     lib.ce_engine_last_sim_jit.restype = ctypes.c_int
+    lib.ce_engine_set_profiling.argtypes = [vp, ctypes.c_int]
+    lib.ce_engine_set_profiling.restype = ctypes.c_int
+    lib.ce_engine_kernel_times.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
+    lib.ce_engine_kernel_times.restype = ctypes.c_int
     lib.ce_detmath_host.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int]
     lib.ce_detmath_host.restype = ctypes.c_int
     lib.ce_detmath_device.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]

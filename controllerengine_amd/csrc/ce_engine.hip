@@ -59,6 +59,11 @@ struct ce_engine {
   };
   Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
   Buf mr32, refined, rlist, rcount, rslots, rcnt, atbp;
+  // per-kernel timing (ce_engine_set_profiling)
+  int profiling = 0;
+  std::vector<hipEvent_t> mark_ev;
+  std::vector<const char*> mark_name;
+  int n_marks = 0;
   float* diag_y = nullptr;
   float* diag_lina = nullptr;
   float* diag_eig = nullptr;
@@ -106,6 +111,41 @@ static const ProblemDev* upload_problem(ce_engine* E, const std::vector<uint8_t>
   E->blob = blob;
   return reinterpret_cast<const ProblemDev*>(E->d_prob);
 }
+
+namespace ce {
+KernelMarker& kernel_marker() {
+  static thread_local KernelMarker m;
+  return m;
+}
+}  // namespace ce
+
+static void record_mark(void* ctx, const char* name, hipStream_t st) {
+  ce_engine* E = static_cast<ce_engine*>(ctx);
+  if (E->n_marks == (int)E->mark_ev.size()) {
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev));
+    E->mark_ev.push_back(ev);
+    E->mark_name.push_back(nullptr);
+  }
+  HIPCHK(hipEventRecord(E->mark_ev[E->n_marks], st));
+  E->mark_name[E->n_marks] = name;
+  E->n_marks++;
+}
+
+// Restores the thread's kernel marker when evaluate() returns or throws.
+struct MarkerScope {
+  KernelMarker saved;
+  bool on;
+  MarkerScope(ce_engine* E, hipStream_t st) : saved(kernel_marker()), on(E->profiling != 0) {
+    if (!on) return;
+    E->n_marks = 0;
+    kernel_marker() = KernelMarker{record_mark, E};
+    record_mark(E, "start", st);
+  }
+  ~MarkerScope() {
+    if (on) kernel_marker() = saved;
+  }
+};
 
 // The six-stage chain for one candidate block (reference src/main.cu:467-556, one round).
 static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, int precision, const float* d_gains,
@@ -179,6 +219,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   E->last_stream = st;
 
   // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
+  MarkerScope marks(E, st);
   HIPCHK(hipEventRecord(E->ev[0], st));
   launch_eig_fp64(H.ns, H.nt, P, B, st);
   launch_compact(B, B.d_y != nullptr, st);
@@ -193,6 +234,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     if (jit_get(E->jit, gen_sim_source(H, mixed), &fn, jerr)) {
       if (!jit_launch_sim(fn, (n + 63) / 64, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n))
         throw std::runtime_error("launch of the specialised simulation kernel failed");
+      kmark(mixed ? "ce_sim_f32" : "ce_sim_f64", st);
       E->last_sim_jit = 1;
     } else {
       fn = nullptr;
@@ -247,6 +289,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);
   jit_cache_destroy(E->jit);
+  for (hipEvent_t ev : E->mark_ev) hipEventDestroy(ev);
   if (E->events_ok)
     for (int i = 0; i < N_EVENTS; i++) hipEventDestroy(E->ev[i]);
   delete E;
@@ -290,6 +333,31 @@ extern "C" CE_EXPORT int ce_engine_last_sim_jit(ce_engine* E) { return E ? E->la
 extern "C" CE_EXPORT int ce_detmath_device(int fn, const double* x, const double* y, double* out, int n, void* stream) {
   ce::launch_detmath(fn, x, y, out, n, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+extern "C" CE_EXPORT int ce_engine_set_profiling(ce_engine* E, int on) {
+  if (!E) return 1;
+  E->profiling = on ? 1 : 0;
+  return 0;
+}
+
+extern "C" CE_EXPORT int ce_engine_kernel_times(ce_engine* E, char* names, int names_len, double* ms, int n_max) {
+  if (!E || E->n_marks < 2) return 0;
+  if (hipEventSynchronize(E->mark_ev[E->n_marks - 1]) != hipSuccess) return -1;
+  std::string list;
+  int k = 0;
+  for (int i = 1; i < E->n_marks && k < n_max; i++, k++) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, E->mark_ev[i - 1], E->mark_ev[i]) != hipSuccess) return -1;
+    ms[k] = t;
+    if (!list.empty()) list += ",";
+    list += E->mark_name[i];
+  }
+  if (names && names_len > 0) {
+    std::strncpy(names, list.c_str(), (size_t)names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  return k;
 }
 
 // =================================================================================================

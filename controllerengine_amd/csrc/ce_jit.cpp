@@ -102,7 +102,8 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed) {
   std::ostringstream& o = g.o;
   const std::string T = g.T();
   o << "// generated: closed-loop simulation, ns=" << ns << " nd=" << nd << " nw=" << nw << (mixed ? " float" : " double") << "\n";
-  o << "extern \"C\" __global__ __launch_bounds__(64) void ce_sim(const float* __restrict__ gains, const int* __restrict__ flags,\n"
+  o << "extern \"C\" __global__ __launch_bounds__(64) void " << (mixed ? "ce_sim_f32" : "ce_sim_f64")
+    << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
        "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
        "    float* __restrict__ dy, int n) {\n";
   o << "  const int idx = blockIdx.x * 64 + threadIdx.x;\n  if (idx >= *count) return;\n  const int c = list[idx];\n";
@@ -224,8 +225,9 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
     err = "hipModuleLoadData failed for the simulation kernel";
     return false;
   }
-  if (hipModuleGetFunction(&e.fn, e.mod, "ce_sim") != hipSuccess) {
-    err = "hipModuleGetFunction(ce_sim) failed";
+  const char* kname = src.find("void ce_sim_f32(") != std::string::npos ? "ce_sim_f32" : "ce_sim_f64";
+  if (hipModuleGetFunction(&e.fn, e.mod, kname) != hipSuccess) {
+    err = std::string("hipModuleGetFunction(") + kname + ") failed";
     hipModuleUnload(e.mod);
     return false;
   }

@@ -481,24 +481,27 @@ void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hi
   if (nt <= 4) hipLaunchKernelGGL(k_eig_lin<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
   else if (nt <= 8) hipLaunchKernelGGL(k_eig_lin<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL(k_eig_lin<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  kmark("k_eig_lin", st);
   const int m = 2 * ns;
   if (m <= 4) hipLaunchKernelGGL(k_eig_nom<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
   else if (m <= 8) hipLaunchKernelGGL(k_eig_nom<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL(k_eig_nom<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  kmark("k_eig_nom", st);
 }
 
 void launch_compact(const StageBufs& B, int all, hipStream_t st) {
   hipLaunchKernelGGL(k_compact, dim3((B.n + 63) / 64), dim3(64), 0, st, B, all);
+  kmark("k_compact", st);
 }
 
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int nb = (2 * nt + 7) / 8;
   const dim3 grid((B.n + 3) / 4);
   switch (nb) {
-    case 1: hipLaunchKernelGGL(k_fit_fp64<1>, grid, dim3(64), 0, st, P, B); return true;
-    case 2: hipLaunchKernelGGL(k_fit_fp64<2>, grid, dim3(64), 0, st, P, B); return true;
-    case 3: hipLaunchKernelGGL(k_fit_fp64<3>, grid, dim3(64), 0, st, P, B); return true;
-    case 4: hipLaunchKernelGGL(k_fit_fp64<4>, grid, dim3(64), 0, st, P, B); return true;
+    case 1: hipLaunchKernelGGL(k_fit_fp64<1>, grid, dim3(64), 0, st, P, B); kmark("k_fit_fp64", st); return true;
+    case 2: hipLaunchKernelGGL(k_fit_fp64<2>, grid, dim3(64), 0, st, P, B); kmark("k_fit_fp64", st); return true;
+    case 3: hipLaunchKernelGGL(k_fit_fp64<3>, grid, dim3(64), 0, st, P, B); kmark("k_fit_fp64", st); return true;
+    case 4: hipLaunchKernelGGL(k_fit_fp64<4>, grid, dim3(64), 0, st, P, B); kmark("k_fit_fp64", st); return true;
   }
   return false;
 }
@@ -516,7 +519,9 @@ bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, h
     case 8: hipLaunchKernelGGL(k_sweep_fp64<8>, grid, dim3(64), 0, st, P, B); break;
     default: return false;
   }
+  kmark("k_sweep_fp64", st);
   hipLaunchKernelGGL(k_scan_fp64, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  kmark("k_scan_fp64", st);
   return true;
 }
 
@@ -525,6 +530,7 @@ void launch_info_init(const float* in, double* out, size_t count, hipStream_t st
 }
 void launch_info_final(const double* in, float* out, size_t count, hipStream_t st) {
   hipLaunchKernelGGL(k_info_final, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, in, out, count);
+  kmark("k_info_final", st);
 }
 
 }  // namespace ce

@@ -428,12 +428,13 @@ bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, h
 bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int nb = (2 * nt + 7) / 8;
   hipLaunchKernelGGL(k_atb, dim3((B.n + 63) / 64, (nt + ATB_MODES - 1) / ATB_MODES, B.atb_parts), dim3(64), 0, st, P, B);
+  kmark("k_atb", st);
   const dim3 grid((B.n + 3) / 4);
   switch (nb) {
-    case 1: hipLaunchKernelGGL(k_fitcf<1>, grid, dim3(64), 0, st, P, B); return true;
-    case 2: hipLaunchKernelGGL(k_fitcf<2>, grid, dim3(64), 0, st, P, B); return true;
-    case 3: hipLaunchKernelGGL(k_fitcf<3>, grid, dim3(64), 0, st, P, B); return true;
-    case 4: hipLaunchKernelGGL(k_fitcf<4>, grid, dim3(64), 0, st, P, B); return true;
+    case 1: hipLaunchKernelGGL(k_fitcf<1>, grid, dim3(64), 0, st, P, B); kmark("k_fitcf", st); return true;
+    case 2: hipLaunchKernelGGL(k_fitcf<2>, grid, dim3(64), 0, st, P, B); kmark("k_fitcf", st); return true;
+    case 3: hipLaunchKernelGGL(k_fitcf<3>, grid, dim3(64), 0, st, P, B); kmark("k_fitcf", st); return true;
+    case 4: hipLaunchKernelGGL(k_fitcf<4>, grid, dim3(64), 0, st, P, B); kmark("k_fitcf", st); return true;
   }
   return false;
 }
@@ -441,10 +442,15 @@ bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream
 template <int NS>
 static void launch_sweep_mixed_t(int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_sweep32<NS>, dim3(B.n, (F + 63) / 64), dim3(64), 0, st, P, B);
+  kmark("k_sweep32", st);
   hipLaunchKernelGGL(k_mark32, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  kmark("k_mark32", st);
   hipLaunchKernelGGL(k_rgather, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  kmark("k_rgather", st);
   hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
+  kmark("k_refine64", st);
   hipLaunchKernelGGL(k_scan_mixed, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
+  kmark("k_scan_mixed", st);
 }
 
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {

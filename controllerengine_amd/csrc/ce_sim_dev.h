@@ -184,6 +184,7 @@ __global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, St
 template <typename R, int NS, int ND>
 static void launch_sim_t(const ProblemDev* P, const StageBufs& B, R* ybuf, hipStream_t st) {
   hipLaunchKernelGGL((k_sim<R, NS, ND>), dim3((B.n + 63) / 64), dim3(64), 0, st, P, B, ybuf);
+  kmark(sizeof(R) == 8 ? "k_sim_fp64" : "k_sim_f32", st);
 }
 
 template <typename R, int NS>

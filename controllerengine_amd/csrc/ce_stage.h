@@ -1,6 +1,8 @@
 // Per-launch stage buffers shared by the engine and the kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "ce_common.h"
 
 namespace ce {
@@ -49,5 +51,19 @@ struct StageBufs {
   float* d_ata;
   float* d_atb;
 };
+
+// Optional per-kernel timing.  Launchers call kmark(name, stream) after each stage kernel; when
+// the calling thread's engine has profiling on, an event is recorded there, so that the time
+// between two marks is the duration of the kernel named by the second one.
+using KernelMarkFn = void (*)(void* ctx, const char* name, hipStream_t st);
+struct KernelMarker {
+  KernelMarkFn fn = nullptr;
+  void* ctx = nullptr;
+};
+KernelMarker& kernel_marker();  // per host thread
+inline void kmark(const char* name, hipStream_t st) {
+  KernelMarker& m = kernel_marker();
+  if (m.fn) m.fn(m.ctx, name, st);
+}
 
 }  // namespace ce

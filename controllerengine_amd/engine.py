@@ -88,3 +88,15 @@ class Engine:
         buf = (ctypes.c_double * 8)()
         k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)
         return [buf[i] for i in range(k)]
+
+    def set_profiling(self, on=True):
+        _abi.lib().ce_engine_set_profiling(self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        """[(kernel name, ms)] of the last evaluate with profiling on, in launch order."""
+        names = ctypes.create_string_buffer(2048)
+        buf = (ctypes.c_double * 64)()
+        k = _abi.lib().ce_engine_kernel_times(self.h, names, len(names), buf, 64)
+        if k <= 0:
+            return []
+        return list(zip(names.value.decode().split(","), [buf[i] for i in range(k)]))

@@ -142,6 +142,12 @@ CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
 /* 1 when the last ce_evaluate ran the runtime-specialised (hipRTC) simulation kernel.  The engine
  * specialises the per-step code of a problem unless CE_JIT=0 is set in the environment. */
 CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
+/* Per-kernel timing (no counterpart in the reference).  With profiling on, ce_evaluate records an
+ * event after every kernel it launches; ce_engine_kernel_times then returns, for the last call, the
+ * number of kernels k (<= n_max), their durations in ms and their names, comma-separated, in
+ * launch order.  Off by default (the events add small gaps between kernels). */
+CE_EXPORT int ce_engine_set_profiling(ce_engine* eng, int on);
+CE_EXPORT int ce_engine_kernel_times(ce_engine* eng, char* names, int names_len, double* ms, int n_max);
 
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
