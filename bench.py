@@ -23,7 +23,7 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
-F32_KERNELS = ("ce_sim_f32", "k_sim_f32", "k_sweep32")
+F32_KERNELS = ("ce_sim_f32", "k_sim_f32", "k_sweep_mark32")
 
 
 def kernel_flops(problem, info64, d):
@@ -56,7 +56,7 @@ def kernel_flops(problem, info64, d):
         "k_atb": float(np.sum(8.0 * u * tfit)),
         "k_fitcf": solve + float(np.sum(u * (u + 1) / 2)) * 2 * 40,
         "k_fit_fp64": float(np.sum(tfit * (nc * (nc + 1) + 2 * nc + 4 * u + 1) + 2 * T)) + solve,
-        "k_sweep32": F * per_point * swept, "k_sweep_fp64": F * per_point * swept,
+        "k_sweep_mark32": F * per_point * swept, "k_sweep_fp64": F * per_point * swept,
     }
 
 

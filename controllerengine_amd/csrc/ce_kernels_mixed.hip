@@ -9,12 +9,13 @@
 //                  right-hand side sum_t phi(t)(y(t) - yss) runs over the stored trajectory with the
 //                  basis advanced by a double complex recurrence.  Then the same LDL^T solve and
 //                  score as the FP64 chain (src/modalsolve.cu).
-//   margin sweep   k_sweep32: every frequency in float.  k_mark32: per candidate, the intervals whose
-//                  float values show a gain or phase crossing, or lie within a guard band of one,
-//                  and the point of smallest b, are marked.  k_refine64: marked points are recomputed
-//                  with the FP64 sweep point (the FP64 chain's arithmetic).  k_scan_mixed: the
-//                  reference's unwrap / crossing / interpolation scan (src/allmargin.cu:119-180) in
-//                  double over the refined values.
+//   margin sweep   k_sweep_mark32: one wave per candidate, every frequency in float; the points
+//                  whose float values show (or come within a guard band of) a gain or phase crossing,
+//                  a non-finite value, or the smallest b(w) and its band are appended to a dense
+//                  list.  k_refine64: the listed points are recomputed with the FP64 sweep point (the
+//                  FP64 chain's arithmetic).  k_scan32: the reference's unwrap / crossing /
+//                  interpolation scan (src/allmargin.cu:119-180) in double over the refined values,
+//                  one wave per candidate (chunk-wise phase prefix, wave minima).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -214,21 +215,10 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
 }
 
 // ------------------------------------------------------------------------------------------------
-template <int NS>
-__global__ __launch_bounds__(64) void k_sweep32(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x;
-  const int w = blockIdx.y * 64 + threadIdx.x;
-  if (margin_skip(P, B, c)) return;
-  const int F = P->F;
-  if (w >= F) return;
-  if (!tab<int32_t>(P, P->off_pok)[w]) { atomicOr(&B.sweep_fail[c], 1); return; }
-  float m, raw, b;
-  const bool ok = sweep_point<NS, float>(P, B.open_a + (size_t)c * NS * NS, B.gains + (size_t)c * P->cuda_w, w, m, raw, b);
-  float* out = B.mr32 + ((size_t)c * F + w) * 3;
-  out[0] = ok ? m : __builtin_nanf("");
-  out[1] = ok ? raw : __builtin_nanf("");
-  out[2] = ok ? b : __builtin_nanf("");
-}
+// Margin sweep, mixed chain.  One wave per candidate; lane l owns the contiguous frequency chunk
+// [l*KM, l*KM + KM), KM = ceil(F / 64) <= KMAX.  Chunk-wise scans combine in-lane sequential
+// passes with a wave prefix over the lanes' chunk totals.
+constexpr int KMAX = 8;  // F <= 512
 
 __device__ __forceinline__ bool near_odd_pi(float ph) {
   const float x = ph + (float)PI_D;
@@ -236,82 +226,190 @@ __device__ __forceinline__ bool near_odd_pi(float ph) {
   return __builtin_fabsf(x - k * (float)TWO_PI_D) < GUARD_PHASE;
 }
 
-// Marks, per candidate, the sweep points to recompute in FP64 (one lane per candidate).
-__global__ __launch_bounds__(64) void k_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= B.n) return;
-  if (margin_skip(P, B, c) || B.sweep_fail[c]) return;
-  const int F = P->F;
-  const float* mr = B.mr32 + (size_t)c * F * 3;
-  int8_t* rf = B.refined + (size_t)c * F;
-  int nmark = 0;
-  bool overflow = false;
-  int* mine = B.rslots + (size_t)c * RMAX;
-  auto mark = [&](int w) {
-    if (rf[w] != 0) return;
-    if (nmark >= RMAX) { overflow = true; return; }
-    rf[w] = 1;
-    mine[nmark++] = w;
-  };
-  const float PIf = (float)PI_D, TPf = (float)TWO_PI_D;
-  float prevMag = 0.0f, prevPhase = 0.0f, prevRaw = 0.0f, bestb = __builtin_inff();
-  int argb = -1;
-  for (int w = 0; w < F; w++) {
-    const float m = mr[3 * w], raw = mr[3 * w + 1], b = mr[3 * w + 2];
-    const bool fin = __builtin_isfinite(m) && __builtin_isfinite(raw);
-    if (b < bestb) { bestb = b; argb = w; }
-    if (w > 0) {
-      float delta = raw - prevRaw;
-      const bool amb = __builtin_fabsf(__builtin_fabsf(delta) - PIf) < GUARD_PHASE;
-      if (delta > PIf) delta -= TPf;
-      else if (delta < -PIf) delta += TPf;
-      const float ph = prevPhase + delta;
-      const bool gcross = (prevMag - 1.0f) * (m - 1.0f) <= 0.0f;
-      const bool gnear = __builtin_fabsf(prevMag - 1.0f) < GUARD_MAG || __builtin_fabsf(m - 1.0f) < GUARD_MAG;
-      const float ka = __builtin_floorf((prevPhase + PIf) / TPf), kb = __builtin_floorf((ph + PIf) / TPf);
-      const bool pcross = ka != kb;
-      const bool pnear = near_odd_pi(prevPhase) || near_odd_pi(ph);
-      if (gcross || gnear || pcross || pnear || amb || !fin) {
-        mark(w - 1);
-        mark(w);
-      }
-      prevPhase = ph;
-    } else {
-      prevPhase = raw;
-      if (!fin) mark(0);
-    }
-    prevMag = m;
-    prevRaw = raw;
-  }
-  // the smallest b(w) and every point whose float b lies within a band of it
-  if (argb >= 0) {
-    mark(argb);
-    const float band = bestb + (1e-3f + 1e-2f * bestb);
-    for (int w = 0; w < F; w++)
-      if (mr[3 * w + 2] <= band) mark(w);
-  }
-  B.rcnt[c] = nmark;
-  if (overflow) atomicOr(&B.flags[c], F_REFINE_OVERFLOW);
-}
-
-// Gathers the per-candidate refinement slots into one dense list: one atomic per wave.
-__global__ __launch_bounds__(64) void k_rgather(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  int cnt = 0;
-  if (c < B.n && !margin_skip(P, B, c) && !B.sweep_fail[c]) cnt = B.rcnt[c];
-  // inclusive scan of cnt over the wave
-  int incl = cnt;
+template <typename V>
+__device__ __forceinline__ V wave_excl_sum(V v, int lane) {
+  V incl = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const int v = __shfl_up(incl, d);
-    if ((int)threadIdx.x >= d) incl += v;
+    const V u = __shfl_up(incl, d);
+    if (lane >= d) incl = incl + u;
   }
-  const int total = __shfl(incl, 63);
+  return incl - v;
+}
+
+// Float sweep of every frequency, then the points whose float values show (or come within a guard
+// band of) a gain or phase crossing, a non-finite value, or the smallest b(w) (and those within a
+// band of it) are marked for FP64 refinement and appended to the dense refinement list, at most
+// RMAX per candidate (crossing points first).
+template <int NS>
+__global__ __launch_bounds__(64) void k_sweep_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x;
+  if (margin_skip(P, B, c)) return;
+  const int F = P->F, lane = threadIdx.x;
+  const int KM = (F + 63) / 64;
+  const int w0 = lane * KM;
+  const int32_t* pok = tab<int32_t>(P, P->off_pok);
+  const double* oa = B.open_a + (size_t)c * NS * NS;
+  const float* gg = B.gains + (size_t)c * P->cuda_w;
+  bool failp = false;
+  float* mr = B.mr32 + (size_t)c * F * 3;
+  const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
+  // the sweep points go to memory and are read back below, so that nothing else is live across
+  // the register-heavy elimination
+#pragma unroll 1
+  for (int k = 0; k < nk; k++) {
+    const int w = w0 + k;
+    float mm = __builtin_nanf(""), rr = __builtin_nanf(""), b2 = __builtin_nanf("");
+    if (!pok[w]) {
+      failp = true;
+    } else {
+      float m1, r1, bq;
+      if (sweep_point<NS, float>(P, oa, gg, w, m1, r1, bq)) { mm = m1; rr = r1; b2 = bq; }
+    }
+    mr[3 * w] = mm;
+    mr[3 * w + 1] = rr;
+    mr[3 * w + 2] = b2;
+  }
+  if (__ballot(failp)) {
+    if (lane == 0) B.sweep_fail[c] = 1;
+    return;
+  }
+  float m[KMAX], raw[KMAX], bb[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    m[k] = raw[k] = bb[k] = __builtin_nanf("");
+    if (k < nk) {
+      const int w = w0 + k;
+      m[k] = mr[3 * w];
+      raw[k] = mr[3 * w + 1];
+      bb[k] = mr[3 * w + 2];
+    }
+  }
+  const float PIf = (float)PI_D, TPf = (float)TWO_PI_D;
+  // unwrapped float phase: ph[w] = raw[0] + sum_{v=1..w} wrap(raw[v] - raw[v-1])
+  float lastRaw = raw[0], lastM = m[0];
+#pragma unroll
+  for (int k = 1; k < KMAX; k++)
+    if (k < nk) { lastRaw = raw[k]; lastM = m[k]; }
+  const float prevRaw0 = __shfl_up(lastRaw, 1);
+  const float prevM0 = __shfl_up(lastM, 1);
+  float dl[KMAX];
+  float loc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    dl[k] = 0.0f;
+    const int w = w0 + k;
+    if (k < nk && w > 0) {
+      float d = raw[k] - (k == 0 ? prevRaw0 : raw[k > 0 ? k - 1 : 0]);
+      if (d > PIf) d -= TPf;
+      else if (d < -PIf) d += TPf;
+      dl[k] = d;
+      loc = loc + d;
+    }
+  }
+  const float off = wave_excl_sum(loc, lane) + __shfl(raw[0], 0);
+  float ph[KMAX];
+  float run = off;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    run = run + dl[k];
+    ph[k] = run;
+  }
+  float lastPh = ph[0];
+#pragma unroll
+  for (int k = 1; k < KMAX; k++)
+    if (k < nk) lastPh = ph[k];
+  const float prevPh0 = __shfl_up(lastPh, 1);
+  // crossing marks: interval (w-1, w) flagged -> mark both ends
+  unsigned cross = 0;
+  bool send_prev = false;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    if (k >= nk) continue;
+    const int w = w0 + k;
+    const bool fin = __builtin_isfinite(m[k]) && __builtin_isfinite(raw[k]);
+    if (w == 0) {
+      if (!fin) cross |= 1u << k;
+      continue;
+    }
+    const float pm_ = (k == 0) ? prevM0 : m[k > 0 ? k - 1 : 0];
+    const float pr_ = (k == 0) ? prevRaw0 : raw[k > 0 ? k - 1 : 0];
+    const float pp_ = (k == 0) ? prevPh0 : ph[k > 0 ? k - 1 : 0];
+    const float delta = raw[k] - pr_;
+    const bool amb = __builtin_fabsf(__builtin_fabsf(delta) - PIf) < GUARD_PHASE;
+    const bool gcross = (pm_ - 1.0f) * (m[k] - 1.0f) <= 0.0f;
+    const bool gnear = __builtin_fabsf(pm_ - 1.0f) < GUARD_MAG || __builtin_fabsf(m[k] - 1.0f) < GUARD_MAG;
+    const float ka = __builtin_floorf((pp_ + PIf) / TPf), kb = __builtin_floorf((ph[k] + PIf) / TPf);
+    const bool pcross = ka != kb;
+    const bool pnear = near_odd_pi(pp_) || near_odd_pi(ph[k]);
+    if (gcross || gnear || pcross || pnear || amb || !fin) {
+      cross |= 1u << k;
+      if (k == 0) send_prev = true;
+      else cross |= 1u << (k - 1);
+    }
+  }
+  const bool recv = __shfl_down(send_prev ? 1 : 0, 1) != 0 && lane < 63;
+  if (recv && nk > 0) cross |= 1u << (nk - 1);
+  // smallest b (first index on ties; NaN never wins) and the band around it
+  float bv = __builtin_inff();
+  int bi = 0x7fffffff;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++)
+    if (k < nk && bb[k] < bv) { bv = bb[k]; bi = w0 + k; }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const float ov = __shfl_xor(bv, d);
+    const int oi = __shfl_xor(bi, d);
+    if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  unsigned band = 0;
+  if (bi != 0x7fffffff) {
+    const float lim = bv + (1e-3f + 1e-2f * bv);
+#pragma unroll
+    for (int k = 0; k < KMAX; k++)
+      if (k < nk && bb[k] <= lim) band |= 1u << k;
+    if (bi >= w0 && bi < w0 + nk) cross |= 1u << (bi - w0);
+  }
+  band &= ~cross;
+  // keep at most RMAX marks: crossing points first, then band points, each in frequency order
+  const int nc1 = __popc(cross);
+  const int pre1 = wave_excl_sum(nc1, lane);
+  const int tot1 = __shfl(pre1 + nc1, 63);
+  unsigned keep = 0;
+  int kept = 0;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++)
+    if ((cross >> k) & 1u) {
+      if (pre1 + kept < RMAX) keep |= 1u << k;
+      kept++;
+    }
+  const int budget = RMAX - tot1;
+  const int nc2 = __popc(band);
+  const int pre2 = wave_excl_sum(nc2, lane);
+  int kept2 = 0;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++)
+    if ((band >> k) & 1u) {
+      if (pre2 + kept2 < budget) keep |= 1u << k;
+      kept2++;
+    }
+  if (lane == 0 && tot1 > RMAX) atomicOr(&B.flags[c], F_REFINE_OVERFLOW);
+  // refinement list (one atomic per wave) and the per-point state
+  const int nkeep = __popc(keep);
+  const int pre = wave_excl_sum(nkeep, lane);
+  const int tot = __shfl(pre + nkeep, 63);
   int base = 0;
-  if (threadIdx.x == 63 && total > 0) base = atomicAdd(B.rcount, total);
-  base = __shfl(base, 63);
-  const int start = base + incl - cnt;
-  for (int k = 0; k < cnt; k++) B.rlist[start + k] = make_int2(c, B.rslots[(size_t)c * RMAX + k]);
+  if (lane == 0 && tot > 0) base = atomicAdd(B.rcount, tot);
+  base = __shfl(base, 0);
+  int8_t* rf = B.refined + (size_t)c * F;
+  int r = 0;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    if (k >= nk) continue;
+    const bool mk = (keep >> k) & 1u;
+    rf[w0 + k] = mk ? 1 : 0;
+    if (mk) B.rlist[base + pre + (r++)] = make_int2(c, w0 + k);
+  }
 }
 
 template <int NS>
@@ -331,84 +429,144 @@ __global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ 
   }
 }
 
-// Crossing scan over the refined values (src/allmargin.cu:119-180), one candidate per lane.
-__global__ __launch_bounds__(64) void k_scan_mixed(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= B.n) return;
+// Crossing scan (src/allmargin.cu:119-180) over the float values with the FP64 refinements in
+// place, one wave per candidate: each interval's contributions are independent once the phase is
+// unwrapped (chunk-wise prefix in double); the minima and flags reduce over the wave.
+__global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
   double* row = B.info + (size_t)c * P->info_w;
   double* sc = row + P->info_scoring;
   double* am = row + P->info_allmargin;
   const double UE = P->unstable_err, EF = P->eig_fail_err;
   const int flags = B.flags[c];
   if (sc[0] >= UE) {
-    sc[1] = sc[0]; sc[2] = sc[0];
-    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    if (lane == 0) {
+      sc[1] = sc[0]; sc[2] = sc[0];
+      am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    }
     return;
   }
   if (flags & F_NOMINAL_UNSTABLE) {
-    sc[1] = sc[0];
-    sc[2] = UE;
-    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
-    const double total = cedm::log((sc[0] + UE) + EPS30);
-    sc[0] = cedm::isfinite_(total) ? total : EF;
+    if (lane == 0) {
+      sc[1] = sc[0];
+      sc[2] = UE;
+      am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+      const double total = cedm::log((sc[0] + UE) + EPS30);
+      sc[0] = cedm::isfinite_(total) ? total : EF;
+    }
     return;
   }
   const int F = P->F;
-  const double ts = P->ts;
-  const double* th = tab<double>(P, P->off_theta);
-  const float* mr = B.mr32 + (size_t)c * F * 3;
-  const double* m64 = B.mag_raw + (size_t)c * F * 3;
-  const int8_t* rf = B.refined + (size_t)c * F;
+  const int KM = (F + 63) / 64;
+  const int w0 = lane * KM;
+  const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;
   bool fail = B.sweep_fail[c] != 0;
-  double PM = FLTMAX, GM = FLTMAX, DM = FLTMAX, Bmin = 1.0;
-  bool crossed = false;
-  double prevMag = 0.0, prevPhase = 0.0, prevRaw = 0.0, prevTheta = 0.0;
-  for (int w = 0; w < F && !fail; w++) {
-    double m, raw, b;
-    const int r = rf[w];
-    if (r == 3) { fail = true; break; }
-    if (r == 2) { m = m64[3 * w]; raw = m64[3 * w + 1]; b = m64[3 * w + 2]; }
-    else { m = (double)mr[3 * w]; raw = (double)mr[3 * w + 1]; b = (double)mr[3 * w + 2]; }
-    if (!(cedm::isfinite_(m) && cedm::isfinite_(raw))) { fail = true; break; }
-    Bmin = cedm::fmin_(Bmin, b);
-    const double theta_w = th[w];
-    if (w > 0) {
-      double delta = raw - prevRaw;
-      if (delta > PI_D) delta = delta - TWO_PI_D;
-      else if (delta < -PI_D) delta = delta + TWO_PI_D;
-      const double ph = prevPhase + delta;
-      if ((prevMag - 1.0) * (m - 1.0) <= 0.0 && m != prevMag) {
-        const double frac = (1.0 - prevMag) / (m - prevMag);
-        const double phase_c = prevPhase + frac * (ph - prevPhase);
-        crossed = true;
-        double pm_deg = 180.0 + phase_c * RAD2DEG;
-        pm_deg = pm_deg - 360.0 * cedm::floor_((pm_deg + 180.0) / 360.0);
-        PM = cedm::fmin_(PM, pm_deg);
-        const double theta_c = prevTheta + frac * (theta_w - prevTheta);
-        const double pm_rad = pm_deg * DEG2RAD;
-        if (theta_c > EPS6 && pm_rad >= 0.0) DM = cedm::fmin_(DM, ((pm_rad / theta_c) * ts) * 1000.0);
-      }
-      const double a = prevPhase + PI_D, bb = ph + PI_D;
-      const double ka = cedm::floor_(a / TWO_PI_D), kb = cedm::floor_(bb / TWO_PI_D);
-      if (ka != kb && bb != a) {
-        const double target = TWO_PI_D * (bb > a ? kb : ka);
-        const double frac = (target - a) / (bb - a);
-        const double mag_p = prevMag + frac * (m - prevMag);
-        GM = cedm::fmin_(GM, -20.0 * cedm::log10(mag_p));
-      }
-      prevPhase = ph;
-    } else {
-      prevPhase = raw;
+  double m[KMAX], raw[KMAX], bb[KMAX];
+  bool failp = false;
+  if (!fail) {
+    const float* mr = B.mr32 + (size_t)c * F * 3;
+    const double* m64 = B.mag_raw + (size_t)c * F * 3;
+    const int8_t* rf = B.refined + (size_t)c * F;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      m[k] = raw[k] = bb[k] = 0.0;
+      if (k >= nk) continue;
+      const int w = w0 + k;
+      const int r = rf[w];
+      if (r == 3) failp = true;
+      if (r == 2) { m[k] = m64[3 * w]; raw[k] = m64[3 * w + 1]; bb[k] = m64[3 * w + 2]; }
+      else { m[k] = (double)mr[3 * w]; raw[k] = (double)mr[3 * w + 1]; bb[k] = (double)mr[3 * w + 2]; }
+      if (!(cedm::isfinite_(m[k]) && cedm::isfinite_(raw[k]))) failp = true;
     }
-    prevMag = m;
-    prevRaw = raw;
-    prevTheta = theta_w;
   }
-  if (fail) {
-    sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
-    am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+  if (fail || __ballot(failp)) {
+    if (lane == 0) {
+      sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
+      am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+    }
     return;
   }
+  const double ts = P->ts;
+  const double* th = tab<double>(P, P->off_theta);
+  double Bmin = 1.0;
+  double lastRaw = raw[0], lastM = m[0];
+#pragma unroll
+  for (int k = 0; k < KMAX; k++)
+    if (k < nk) {
+      Bmin = cedm::fmin_(Bmin, bb[k]);
+      lastRaw = raw[k];
+      lastM = m[k];
+    }
+  const double prevRaw0 = __shfl_up(lastRaw, 1);
+  const double prevM0 = __shfl_up(lastM, 1);
+  double dl[KMAX];
+  double loc = 0.0;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    dl[k] = 0.0;
+    const int w = w0 + k;
+    if (k < nk && w > 0) {
+      double d = raw[k] - (k == 0 ? prevRaw0 : raw[k > 0 ? k - 1 : 0]);
+      if (d > PI_D) d = d - TWO_PI_D;
+      else if (d < -PI_D) d = d + TWO_PI_D;
+      dl[k] = d;
+      loc = loc + d;
+    }
+  }
+  const double off = wave_excl_sum(loc, lane) + __shfl(raw[0], 0);
+  double ph[KMAX];
+  double run = off;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    run = run + dl[k];
+    ph[k] = run;
+  }
+  double lastPh = ph[0];
+#pragma unroll
+  for (int k = 1; k < KMAX; k++)
+    if (k < nk) lastPh = ph[k];
+  const double prevPh0 = __shfl_up(lastPh, 1);
+  double PM = FLTMAX, GM = FLTMAX, DM = FLTMAX;
+  bool crossed = false;
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    if (k >= nk) continue;
+    const int w = w0 + k;
+    if (w == 0) continue;
+    const double prevMag = (k == 0) ? prevM0 : m[k > 0 ? k - 1 : 0];
+    const double prevPhase = (k == 0) ? prevPh0 : ph[k > 0 ? k - 1 : 0];
+    const double prevTheta = th[w - 1], theta_w = th[w];
+    const double mk = m[k], phk = ph[k];
+    if ((prevMag - 1.0) * (mk - 1.0) <= 0.0 && mk != prevMag) {
+      const double frac = (1.0 - prevMag) / (mk - prevMag);
+      const double phase_c = prevPhase + frac * (phk - prevPhase);
+      crossed = true;
+      double pm_deg = 180.0 + phase_c * RAD2DEG;
+      pm_deg = pm_deg - 360.0 * cedm::floor_((pm_deg + 180.0) / 360.0);
+      PM = cedm::fmin_(PM, pm_deg);
+      const double theta_c = prevTheta + frac * (theta_w - prevTheta);
+      const double pm_rad = pm_deg * DEG2RAD;
+      if (theta_c > EPS6 && pm_rad >= 0.0) DM = cedm::fmin_(DM, ((pm_rad / theta_c) * ts) * 1000.0);
+    }
+    const double a = prevPhase + PI_D, b2 = phk + PI_D;
+    const double ka = cedm::floor_(a / TWO_PI_D), kb = cedm::floor_(b2 / TWO_PI_D);
+    if (ka != kb && b2 != a) {
+      const double target = TWO_PI_D * (b2 > a ? kb : ka);
+      const double frac = (target - a) / (b2 - a);
+      const double mag_p = prevMag + frac * (mk - prevMag);
+      GM = cedm::fmin_(GM, -20.0 * cedm::log10(mag_p));
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    PM = cedm::fmin_(PM, __shfl_xor(PM, d));
+    GM = cedm::fmin_(GM, __shfl_xor(GM, d));
+    DM = cedm::fmin_(DM, __shfl_xor(DM, d));
+    Bmin = cedm::fmin_(Bmin, __shfl_xor(Bmin, d));
+  }
+  crossed = __ballot(crossed) != 0ull;
+  if (lane != 0) return;
   const double err_gm = cedm::fmax_(0.0, P->target_gm - GM) / P->target_gm;
   const double err_pm = crossed ? cedm::fmax_(0.0, P->target_pm - PM) / P->target_pm : 0.0;
   const double err_b = cedm::fmax_(0.0, P->target_b - Bmin) / P->target_b;
@@ -441,19 +599,16 @@ bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream
 
 template <int NS>
 static void launch_sweep_mixed_t(int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  hipLaunchKernelGGL(k_sweep32<NS>, dim3(B.n, (F + 63) / 64), dim3(64), 0, st, P, B);
-  kmark("k_sweep32", st);
-  hipLaunchKernelGGL(k_mark32, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
-  kmark("k_mark32", st);
-  hipLaunchKernelGGL(k_rgather, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
-  kmark("k_rgather", st);
+  hipLaunchKernelGGL(k_sweep_mark32<NS>, dim3(B.n), dim3(64), 0, st, P, B);
+  kmark("k_sweep_mark32", st);
   hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
   kmark("k_refine64", st);
-  hipLaunchKernelGGL(k_scan_mixed, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
-  kmark("k_scan_mixed", st);
+  hipLaunchKernelGGL(k_scan32, dim3(B.n), dim3(64), 0, st, P, B);
+  kmark("k_scan32", st);
 }
 
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  if (F > 64 * KMAX) return false;
   switch (ns) {
     case 1: launch_sweep_mixed_t<1>(F, P, B, st); return true;
     case 2: launch_sweep_mixed_t<2>(F, P, B, st); return true;

@@ -14,7 +14,7 @@ enum : int32_t {
   F_REFINE_OVERFLOW = 8    // mixed sweep: more than RMAX points wanted FP64 refinement
 };
 
-constexpr int RMAX = 32;   // FP64 refinement points per candidate in the mixed sweep
+constexpr int RMAX = 64;   // FP64 refinement points per candidate in the mixed sweep
 
 struct StageBufs {
   int32_t n;                    // candidates in this launch
@@ -36,8 +36,6 @@ struct StageBufs {
   // mixed sweep
   float* mr32;                  // n x F x 3: float |loop|, arg(loop), b(w)
   int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite
-  int32_t* rslots;              // n x RMAX: frequencies to refine, per candidate
-  int32_t* rcnt;                // n: used slots per candidate
   int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
   int32_t* rcount;              // 1
   // mixed fit: partial right-hand sides sum_{t in part} (y_t - yss) z_e^(t - t_part), complex,
