@@ -69,7 +69,8 @@ EXPORTS = [
     "RunControllerEngine", "ce_config_parse", "ce_config_derive", "ce_plant_load_dir", "ce_engine_create",
     "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times",
     "ce_engine_last_sim_jit", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_detmath_host",
-    "ce_detmath_device",
+    "ce_detmath_device", "ce_batch_create", "ce_batch_destroy", "ce_batch_clear", "ce_batch_add", "ce_batch_update",
+    "ce_batch_offsets", "ce_batch_set_options", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
 ]
 
 _lib = None
@@ -107,11 +108,26 @@ def _declare(lib):
     lib.ce_detmath_host.restype = ctypes.c_int
     lib.ce_detmath_device.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
     lib.ce_detmath_device.restype = ctypes.c_int
-    for name in ("ce_batch_create", "ce_batch_destroy", "ce_batch_add", "ce_batch_run", "ce_batch_reset",
-                 "ce_server_create", "ce_server_destroy", "ce_server_submit", "ce_server_poll", "ce_server_step",
-                 "ce_server_stats"):
-        if hasattr(lib, name):
-            getattr(lib, name)  # resolved lazily by the modules that use them
+    lib.ce_batch_create.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_batch_create.restype = vp
+    lib.ce_batch_destroy.argtypes = [vp]
+    lib.ce_batch_destroy.restype = None
+    lib.ce_batch_clear.argtypes = [vp]
+    lib.ce_batch_clear.restype = ctypes.c_int
+    lib.ce_batch_add.argtypes = [vp, P(CeConfig), P(CePlant), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_batch_add.restype = ctypes.c_int
+    lib.ce_batch_update.argtypes = [vp, ctypes.c_int, P(CeConfig), P(CePlant), ctypes.c_char_p, ctypes.c_int]
+    lib.ce_batch_update.restype = ctypes.c_int
+    lib.ce_batch_offsets.argtypes = [vp, ctypes.c_int, P(ctypes.c_longlong), P(ctypes.c_longlong)]
+    lib.ce_batch_offsets.restype = ctypes.c_int
+    lib.ce_batch_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.ce_batch_set_options.restype = ctypes.c_int
+    lib.ce_batch_run.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_batch_run.restype = ctypes.c_int
+    lib.ce_batch_info.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
+    lib.ce_batch_info.restype = ctypes.c_int
+    lib.ce_batch_kernel_times.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
+    lib.ce_batch_kernel_times.restype = ctypes.c_int
     return lib
 
 

@@ -28,7 +28,7 @@ bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, h
 bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
-void launch_compact(const StageBufs& B, int all, hipStream_t st);
+void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
 }  // namespace ce
@@ -218,7 +218,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   MarkerScope marks(E, st);
   HIPCHK(hipEventRecord(E->ev[0], st));
   launch_eig_fp64(H.ns, H.nt, P, B, st);
-  launch_compact(B, B.d_y != nullptr, st);
+  launch_compact(P, B, st);
   HIPCHK(hipEventRecord(E->ev[1], st));
   // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
   void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;

@@ -15,6 +15,7 @@
 #include <hip/hiprtc.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -57,7 +58,9 @@ struct Gen {
 
   // One instruction chain (reference src/main.cu:260-291) as straight-line code; returns the
   // variable holding the chain value.  src: names of the source entries; seed: name or "".
-  std::string chain(const Ins* ops, int count, const std::vector<std::string>& src, const std::string& seed) {
+  // cst(r, which, value): the expression of instruction r's operand constant (which: 0 a, 1 b).
+  template <typename Cst>
+  std::string chain(const Ins* ops, int count, const std::vector<std::string>& src, const std::string& seed, Cst cst) {
     const std::string res = "r" + std::to_string(tmp++);
     o << "    " << T() << " " << res << " = " << zero() << ";\n";
     bool started = false;
@@ -74,10 +77,10 @@ struct Gen {
       } else {
         prev = pos;
       }
-      auto opnd = [&](int kind, double val) {
-        return kind == K_CONST ? L(val) : (kind == K_STATE ? pos : prev);
+      auto opnd = [&](int kind, int which, double val) {
+        return kind == K_CONST ? cst(r, which, val) : (kind == K_STATE ? pos : prev);
       };
-      const std::string a = opnd(in.akind, in.aval), b = opnd(in.bkind, in.bval);
+      const std::string a = opnd(in.akind, 0, in.aval), b = opnd(in.bkind, 1, in.bval);
       switch (in.op) {
         case 1: o << "    " << res << " = " << a << " + " << b << ";\n"; break;
         case 2: o << "    " << res << " = " << a << " - " << b << ";\n"; break;
@@ -95,74 +98,169 @@ struct Gen {
 
 }  // namespace
 
-std::string gen_sim_source(const ProblemDev& P, bool mixed) {
-  Gen g;
-  g.f32 = mixed;
+// Per-candidate body shared by both forms.  Literal form: every constant and size is written into
+// the source.  Structural form: only the dimensions and the instruction structure (opcodes,
+// columns, operand kinds) are fixed; constants and sizes are loaded from the problem descriptor
+// (Pp, byte offsets into ProblemDev), so one compiled kernel serves every problem of that structure.
+static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   const int ns = P.ns, nd = P.nd, nw = P.nw;
   std::ostringstream& o = g.o;
   const std::string T = g.T();
-  o << "// generated: closed-loop simulation, ns=" << ns << " nd=" << nd << " nw=" << nw << (mixed ? " float" : " double") << "\n";
-  o << "extern \"C\" __global__ __launch_bounds__(64) void " << (mixed ? "ce_sim_f32" : "ce_sim_f64")
-    << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
-       "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
-       "    float* __restrict__ dy, int n) {\n";
-  o << "  const int idx = blockIdx.x * 64 + threadIdx.x;\n  if (idx >= *count) return;\n  const int c = list[idx];\n";
-  o << "  const bool unstable = (flags[c] & 1) != 0;\n";
-  o << "  const float* g = gains + (long)c * " << P.cuda_w << ";\n";
+  // constants inside the time loop are read through Pq, which an empty asm statement redefines
+  // (in scalar registers) every step: the loads stay scalar loads from the scalar cache instead of
+  // being hoisted into vector registers for the whole loop
+  bool in_loop = false;
+  // loads through the constant address space: read-only, so uniform addresses become scalar loads
+  auto ld = [&](const char* ty, size_t off) {
+    return std::string("(*(const __attribute__((address_space(4))) ") + ty + "*)((const __attribute__((address_space(4))) char*)" +
+           (in_loop ? "Pq" : "Pp") + " + " + std::to_string(off) + "))";
+  };
+  // plant / observer coefficient: double field (FP64 form) or its float copy (mixed form)
+  auto K = [&](double v, size_t offd, size_t offf) {
+    if (!structural) return g.L(v);
+    return mixed ? ld("float", offf) : ld("double", offd);
+  };
+  auto I = [&](int v, size_t off) { return structural ? ld("int", off) : std::to_string(v); };
+#define CE_OFF(f) offsetof(ProblemDev, f)
+  const std::string sT = I(P.T, CE_OFF(T)), sF0 = I(P.fit_start_id, CE_OFF(fit_start_id));
+  const std::string sFC = I(P.fit_count, CE_OFF(fit_count)), sS0 = I(P.steady_start_id, CE_OFF(steady_start_id));
+  const std::string sRS = I(P.ref_step_id, CE_OFF(ref_step_id)), sCW = I(P.cuda_w, CE_OFF(cuda_w));
+  const std::string sIW = I(P.info_w, CE_OFF(info_w)), sFR = I(P.info_freqresp, CE_OFF(info_freqresp));
+  o << "  const int idx = bx * 64 + threadIdx.x;\n  if (idx >= *A.count) return;\n  const int c = A.list[idx];\n";
+  o << "  const int n = A.n;\n  const int TT = " << sT << ", f0 = " << sF0 << ", f1 = " << sF0 << " + " << sFC
+    << ", s0 = " << sS0 << ", rs = " << sRS << ";\n";
+  o << "  const bool unstable = (A.flags[c] & 1) != 0;\n";
+  o << "  const float* g = A.gains + (long)c * " << sCW << ";\n";
   for (int i = 0; i < ns; i++) o << "  const " << T << " K" << i << " = (" << T << ")g[" << i << "], L" << i << " = (" << T << ")g[" << ns + i << "];\n";
   o << "  const " << T << " N = (" << T << ")g[" << 2 * ns << "];\n";
   for (int i = 0; i < ns; i++) o << "  " << T << " xh" << i << " = " << g.zero() << ";\n";
   for (int i = 0; i < nd; i++) o << "  " << T << " xp" << i << " = " << g.zero() << ";\n";
   o << "  " << T << " ysum = " << g.zero() << ", ymax = -" << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ", ymin = "
     << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ";\n";
-  const int f0 = P.fit_start_id, f1 = P.fit_start_id + P.fit_count, s0 = P.steady_start_id;
-  o << "  " << T << "* yb = (" << T << "*)ybuf + idx;\n";
-  o << "  for (int t = 0; t < " << P.T << "; t++) {\n";
+  o << "  " << T << "* yb = (" << T << "*)A.ybuf + idx;\n";
+  o << "  float* dy = A.dy;\n";
+  o << "  const " << T << " amp = " << K(P.ref_amp, CE_OFF(ref_amp), CE_OFF(fref_amp)) << ";\n";
+  if (structural) o << "  const char* Pq = Pp;\n";
+  o << "  for (int t = 0; t < TT; t++) {\n";
+  if (structural)
+    o << "    asm volatile(\"\" : \"+s\"(Pq));\n"
+         "    Pq = (const char*)(((unsigned long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long)Pq >> 32)) << 32) |\n"
+         "                       (unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)(unsigned long)Pq));\n";
+  in_loop = true;
   std::vector<std::string> sxp, sxh, snl;
   for (int i = 0; i < nd; i++) sxp.push_back("xp" + std::to_string(i));
   for (int i = 0; i < ns; i++) sxh.push_back("xh" + std::to_string(i));
-  const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "");
+  auto ins_cst = [&](const Ins* ops, size_t off_ins, size_t off_fa, size_t off_fb) {
+    return [&, ops, off_ins, off_fa, off_fb](int r, int which, double v) {
+      if (!structural) return g.L(v);
+      if (mixed) return ld("float", (which ? off_fb : off_fa) + (size_t)r * sizeof(float));
+      return ld("double", off_ins + (size_t)r * sizeof(Ins) + (which ? offsetof(Ins, bval) : offsetof(Ins, aval)));
+    };
+  };
+  const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "", ins_cst(P.dc_ops, CE_OFF(dc_ops), CE_OFF(fdc_aval), CE_OFF(fdc_bval)));
   o << "    const " << T << " y = " << y << ";\n";
-  o << "    if (dy) dy[(long)c * " << P.T << " + t] = (float)y;\n";
-  o << "    if (t >= " << f0 << " && t < " << f1 << ") yb[(long)(t - " << f0 << ") * n] = y;\n";
-  o << "    if (t >= " << s0 << ") ysum = ysum + y;\n";
+  o << "    if (dy) dy[(long)c * TT + t] = (float)y;\n";
+  o << "    if (t >= f0 && t < f1) yb[(long)(t - f0) * n] = y;\n";
+  o << "    if (t >= s0) ysum = ysum + y;\n";
   o << "    ymax = " << g.fmax() << "(ymax, y); ymin = " << g.fmin() << "(ymin, y);\n";
   o << "    " << T << " kx = " << g.zero() << ";\n";
   for (int k = 0; k < ns; k++) o << "    kx = " << g.fma() << "(K" << k << ", xh" << k << ", kx);\n";
-  o << "    const " << T << " rr = (t < " << P.ref_step_id << ") ? " << g.zero() << " : " << g.L(P.ref_amp) << ";\n";
+  o << "    const " << T << " rr = (t < rs) ? " << g.zero() << " : amp;\n";
   o << "    const " << T << " u_raw = " << g.fma() << "(N, rr, -kx);\n";
-  const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw");
+  const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", ins_cst(P.u_ops, CE_OFF(u_ops), CE_OFF(fu_aval), CE_OFF(fu_bval)));
   o << "    const " << T << " uu = " << u << ";\n";
   o << "    " << T << " cx = " << g.zero() << ";\n";
-  for (int k = 0; k < ns; k++) o << "    cx = " << g.fma() << "(" << g.L(P.sc[k]) << ", xh" << k << ", cx);\n";
-  o << "    const " << T << " yerr = y - " << g.fma() << "(" << g.L(P.sd) << ", uu, cx);\n";
+  for (int k = 0; k < ns; k++)
+    o << "    cx = " << g.fma() << "(" << K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4) << ", xh" << k << ", cx);\n";
+  o << "    const " << T << " yerr = y - " << g.fma() << "(" << K(P.sd, CE_OFF(sd), CE_OFF(fsd)) << ", uu, cx);\n";
   for (int i = 0; i < ns; i++) {
-    o << "    " << T << " dh" << i << " = " << g.fma() << "(L" << i << ", yerr, " << g.L(P.sb[i]) << " * uu);\n";
-    for (int k = 0; k < ns; k++) o << "    dh" << i << " = " << g.fma() << "(" << g.L(P.sai[i * ns + k]) << ", xh" << k << ", dh" << i << ");\n";
+    o << "    " << T << " dh" << i << " = " << g.fma() << "(L" << i << ", yerr, " << K(P.sb[i], CE_OFF(sb) + i * 8, CE_OFF(fsb) + i * 4) << " * uu);\n";
+    for (int k = 0; k < ns; k++)
+      o << "    dh" << i << " = " << g.fma() << "(" << K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4)
+        << ", xh" << k << ", dh" << i << ");\n";
   }
   for (int i = 0; i < ns; i++) o << "    xh" << i << " = xh" << i << " + dh" << i << ";\n";
   snl = sxp;
   snl.push_back("y");
   std::vector<std::string> W;
-  for (int q = 0; q < nw; q++) W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, ""));
+  for (int q = 0; q < nw; q++) {
+    const size_t oi = CE_OFF(nl_ops) + (size_t)q * CE_MAX_NL_INSTRUCTIONS * sizeof(Ins);
+    const size_t ofa = CE_OFF(fnl_aval) + (size_t)q * CE_MAX_NL_INSTRUCTIONS * sizeof(float);
+    const size_t ofb = CE_OFF(fnl_bval) + (size_t)q * CE_MAX_NL_INSTRUCTIONS * sizeof(float);
+    W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, "", ins_cst(P.nl_ops[q], oi, ofa, ofb)));
+  }
   for (int i = 0; i < nd; i++) {
-    o << "    " << T << " ph" << i << " = " << g.L(P.dbu[i]) << " * uu;\n";
-    for (int q = 0; q < nw; q++) o << "    ph" << i << " = " << g.fma() << "(" << g.L(P.dbw[i * nw + q]) << ", " << W[q] << ", ph" << i << ");\n";
-    for (int k = 0; k < nd; k++) o << "    ph" << i << " = " << g.fma() << "(" << g.L(P.dai[i * nd + k]) << ", xp" << k << ", ph" << i << ");\n";
+    o << "    " << T << " ph" << i << " = " << K(P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4) << " * uu;\n";
+    for (int q = 0; q < nw; q++)
+      o << "    ph" << i << " = " << g.fma() << "(" << K(P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4)
+        << ", " << W[q] << ", ph" << i << ");\n";
+    for (int k = 0; k < nd; k++)
+      o << "    ph" << i << " = " << g.fma() << "(" << K(P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4)
+        << ", xp" << k << ", ph" << i << ");\n";
   }
   for (int i = 0; i < nd; i++) o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
   o << "  }\n";
+  in_loop = false;
   o << "  if (unstable) return;\n";
   // yss and overshoot (reference src/modalsetup.cu:52-62)
-  o << "  const " << T << " yss = ysum / (" << T << ")" << (P.T - s0) << ";\n";
+  o << "  const " << T << " yss = ysum / (" << T << ")(TT - s0);\n";
   o << "  const " << T << " pk = " << g.fmax() << "(" << g.zero() << ", ((yss < " << g.zero() << ") ? ymin : ymax) * yss);\n";
-  o << "  double* fr = info + (long)c * " << P.info_w << " + " << P.info_freqresp << ";\n";
+  o << "  double* fr = A.info + (long)c * " << sIW << " + " << sFR << ";\n";
   if (!mixed) {
     o << "  fr[2] = pk / __builtin_fmax(yss * yss, " << lit(EPS6) << ") - 1.0;\n  fr[3] = yss;\n";
   } else {
     o << "  fr[2] = (double)(pk / __builtin_fmaxf(yss * yss, " << litf((float)EPS6) << ") - 1.0f);\n  fr[3] = (double)yss;\n";
   }
+#undef CE_OFF
+}
+
+static const char* kSimArgs =
+    "struct SimArgs { const float* gains; const int* flags; const int* list; const int* count; void* ybuf;\n"
+    "  double* info; float* dy; int n; int pad; };\n";
+
+std::string gen_sim_source(const ProblemDev& P, bool mixed) {
+  Gen g;
+  g.f32 = mixed;
+  std::ostringstream& o = g.o;
+  const char* name = mixed ? "ce_sim_f32" : "ce_sim_f64";
+  o << "// kernel: " << name << "\n";
+  o << "// generated: closed-loop simulation, ns=" << P.ns << " nd=" << P.nd << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
+  o << kSimArgs;
+  o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
+  gen_body(g, P, mixed, false);
   o << "}\n";
+  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
+    << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
+       "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
+       "    float* __restrict__ dy, int n) {\n"
+       "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n"
+       "  sim_one(A, nullptr, blockIdx.x);\n}\n";
+  return o.str();
+}
+
+std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
+  Gen g;
+  g.f32 = mixed;
+  std::ostringstream& o = g.o;
+  const char* name = mixed ? "ce_simb_f32" : "ce_simb_f64";
+  o << "// kernel: " << name << "\n";
+  o << "// generated: closed-loop simulation for every problem of one structure, ns=" << P.ns << " nd=" << P.nd
+    << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
+  o << kSimArgs;
+  o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
+  gen_body(g, P, mixed, true);
+  o << "}\n";
+  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
+    << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
+       "    const char* const* __restrict__ probs, int J, int total) {\n"
+       "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
+       "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
+       "    int lo = 0, hi = J - 1;\n"
+       "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
+       "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
+       "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
+       "    sim_one(args[p], probs[p], lb);\n"
+       "  }\n}\n";
   return o.str();
 }
 
@@ -225,8 +323,10 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
     err = "hipModuleLoadData failed for the simulation kernel";
     return false;
   }
-  const char* kname = src.find("void ce_sim_f32(") != std::string::npos ? "ce_sim_f32" : "ce_sim_f64";
-  if (hipModuleGetFunction(&e.fn, e.mod, kname) != hipSuccess) {
+  // the first line names the kernel: "// kernel: NAME"
+  const size_t nl = src.find('\n');
+  const std::string kname = src.substr(11, nl - 11);
+  if (hipModuleGetFunction(&e.fn, e.mod, kname.c_str()) != hipSuccess) {
     err = std::string("hipModuleGetFunction(") + kname + ") failed";
     hipModuleUnload(e.mod);
     return false;
@@ -240,6 +340,25 @@ bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, cons
                     const int* count, void* ybuf, double* info, float* dy, int n) {
   void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
+}
+
+bool jit_launch_sim_batch(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
+                          const char* const* probs, int J, int total) {
+  void* a[] = {&start, &pid, &args, &probs, &J, &total};
+  return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, a, nullptr) == hipSuccess;
+}
+
+std::string sim_structure_key(const ProblemDev& P, bool mixed) {
+  std::ostringstream k;
+  k << (mixed ? "f" : "d") << P.ns << "." << P.nd << "." << P.nw << "|";
+  auto chain = [&](const Ins* ops, int n) {
+    for (int r = 0; r < n; r++) k << ops[r].op << "," << ops[r].col << "," << ops[r].akind << "," << ops[r].bkind << ";";
+    k << "|";
+  };
+  chain(P.dc_ops, P.dc_n);
+  chain(P.u_ops, P.u_n);
+  for (int q = 0; q < P.nw; q++) chain(P.nl_ops[q], P.nl_n[q]);
+  return k.str();
 }
 
 }  // namespace ce

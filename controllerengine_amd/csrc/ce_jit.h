@@ -13,10 +13,30 @@ struct JitCache;
 JitCache* jit_cache_create();
 void jit_cache_destroy(JitCache* c);
 bool jit_enabled();
-// HIP source of the simulation kernel `ce_sim` for problem P (mixed: float state and trajectory).
+// HIP source of the simulation kernel for problem P (mixed: float state and trajectory), every
+// constant written into the source.
 std::string gen_sim_source(const ProblemDev& P, bool mixed);
 bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err);
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n);
+
+// Batched form: one kernel per instruction structure and dimensions (sim_structure_key), constants
+// and sizes read from each problem's descriptor.  SimArgs mirrors the struct in the generated
+// source.
+struct SimArgs {
+  const float* gains;
+  const int* flags;
+  const int* list;
+  const int* count;
+  void* ybuf;
+  double* info;
+  float* dy;
+  int n;
+  int pad;
+};
+std::string gen_sim_batch_source(const ProblemDev& P, bool mixed);
+std::string sim_structure_key(const ProblemDev& P, bool mixed);
+bool jit_launch_sim_batch(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
+                          const char* const* probs, int J, int total);
 
 }  // namespace ce

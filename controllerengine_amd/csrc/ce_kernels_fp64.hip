@@ -36,8 +36,10 @@ __global__ __launch_bounds__(256) void k_info_final(const double* __restrict__ i
 
 // Compaction: the candidates whose trajectory is needed (linearly stable ones, or all of them when
 // the trajectory dump is requested) get consecutive slots; one atomic per wave.
-__global__ __launch_bounds__(64) void k_compact(StageBufs B, int all) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void k_compact_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx,
+                                               int by, int bz) {
+  const bool all = B.d_y != nullptr;
+  const int c = bx * 64 + threadIdx.x;
   const bool pred = c < B.n && (all || (B.flags[c] & F_UNSTABLE) == 0);
   const unsigned long long ballot = __ballot(pred);
   const int cnt = __popcll(ballot);
@@ -50,6 +52,17 @@ __global__ __launch_bounds__(64) void k_compact(StageBufs B, int all) {
   }
 }
 
+__global__ __launch_bounds__(64) void k_compact(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_compact_body(P, B, blockIdx.x, gridDim.x, 0, 0);
+}
+
+__global__ __launch_bounds__(64) void k_compact_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_compact_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Eigen stage: LIN_A (reference src/eigensetup.cu:16-56), its eigenvalues, the continuous-time
 // modes with the adjacent-duplicate rule (src/modalsetup.cu:8-47), the unstable flag
@@ -59,12 +72,12 @@ __global__ __launch_bounds__(64) void k_compact(StageBufs B, int all) {
 // matrix and owns column / row l in the eigensolver.  k_eig_lin (G >= nt) solves LIN_A and derives
 // modes and flags; k_eig_nom (G >= 2 ns) solves the nominal loop of the candidates that need it.
 template <int G>
-__global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P, StageBufs B) {
+__device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int S = G + 1, CPW = 64 / G;
   __shared__ double smat[CPW][G * S];
   __shared__ double svec[CPW][5 * G];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
-  const int c = blockIdx.x * CPW + gi;
+  const int c = bx * CPW + gi;
   if (c >= B.n) return;
   double* T = smat[gi];
   double* V = svec[gi];
@@ -146,14 +159,28 @@ __global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P
   }
 }
 
+template <int G>
+__global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_eig_lin_body<G>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_eig_lin_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_eig_lin_body<G>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79)
 template <int G>
-__global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P, StageBufs B) {
+__device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int S = G + 1, CPW = 64 / G;
   __shared__ double smat[CPW][G * S];
   __shared__ double svec[CPW][3 * G];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
-  const int c = blockIdx.x * CPW + gi;
+  const int c = bx * CPW + gi;
   if (c >= B.n) return;
   const int flags = B.flags[c];
   if (flags & F_NOMINAL_SKIPPED) return;
@@ -181,6 +208,20 @@ __global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P
   if (l == 0 && bad) B.flags[c] = flags | F_NOMINAL_UNSTABLE;
 }
 
+template <int G>
+__global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_eig_nom_body<G>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_eig_nom_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_eig_nom_body<G>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // ------------------------------------------------------------------------------------------------
 // Modal fit: Gram accumulation (src/modalsetup.cu:64-96) + LDLT solve and behaviour score
 // (src/modalsolve.cu:150-203).  NB = number of 8-column blocks of the 2nt-wide normal equations.
@@ -201,7 +242,7 @@ struct FitShape {
 };
 
 template <int NB>
-__global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
+__device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   using S = FitShape<NB>;
   constexpr int G = S::G, CPW = S::CPW, NCP = S::NCP, NM = S::NM, TB = S::TB;
   __shared__ double phi[CPW][TB][NCP];
@@ -212,7 +253,7 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
 
   const int lane = threadIdx.x;
   const int slot = lane / G, g = lane % G;
-  const int idx = blockIdx.x * CPW + slot;
+  const int idx = bx * CPW + slot;
   const int n = B.n;
   const int nt = P->nt, nc2 = 2 * nt;
   bool active = false;
@@ -370,11 +411,25 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   solve_and_score_grp<G>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g);
 }
 
+template <int NB>
+__global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_fit_fp64_body<NB>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_fit_fp64_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_fit_fp64_body<NB>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // ------------------------------------------------------------------------------------------------
 template <int NS>
-__global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x;
-  const int w = blockIdx.y * 64 + threadIdx.x;
+__device__ __forceinline__ void k_sweep_fp64_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int c = bx;
+  const int w = by * 64 + threadIdx.x;
   if (margin_skip(P, B, c)) return;
   const int F = P->F;
   if (w >= F) return;
@@ -390,9 +445,24 @@ __global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict_
   if (b == b) atomicMin(&B.bmin[c], (unsigned long long)__double_as_longlong(b));
 }
 
+template <int NS>
+__global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_sweep_fp64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_sweep_fp64_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    const StageBufs& B = T.bufs[q.p];
+    k_sweep_fp64_body<NS>(T.probs[q.p], B, q.lb % B.n, B.n, q.lb / B.n, 0);
+  }
+}
+
+
 // Sequential part of the sweep (src/allmargin.cu:119-180), one candidate per lane.
-__global__ __launch_bounds__(64) void k_scan_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void k_scan_fp64_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int c = bx * 64 + threadIdx.x;
   if (c >= B.n) return;
   double* row = B.info + (size_t)c * P->info_w;
   double* sc = row + P->info_scoring;
@@ -470,6 +540,18 @@ __global__ __launch_bounds__(64) void k_scan_fp64(const ProblemDev* __restrict__
   sc[0] = cedm::isfinite_(total) ? total : EF;
 }
 
+__global__ __launch_bounds__(64) void k_scan_fp64(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_scan_fp64_body(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+__global__ __launch_bounds__(64) void k_scan_fp64_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_scan_fp64_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // ------------------------------------------------------------------------------------------------
 // Launch helpers (host side of this translation unit)
 
@@ -489,8 +571,8 @@ void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hi
   kmark("k_eig_nom", st);
 }
 
-void launch_compact(const StageBufs& B, int all, hipStream_t st) {
-  hipLaunchKernelGGL(k_compact, dim3((B.n + 63) / 64), dim3(64), 0, st, B, all);
+void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
   kmark("k_compact", st);
 }
 
@@ -523,6 +605,48 @@ bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, h
   hipLaunchKernelGGL(k_scan_fp64, dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
   kmark("k_scan_fp64", st);
   return true;
+}
+
+bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
+  const dim3 g(grid), b(64);
+#define CE_NS_CASES(K)                                                      \
+  switch (key) {                                                            \
+    case 1: hipLaunchKernelGGL(K<1>, g, b, 0, st, T); return true;          \
+    case 2: hipLaunchKernelGGL(K<2>, g, b, 0, st, T); return true;          \
+    case 3: hipLaunchKernelGGL(K<3>, g, b, 0, st, T); return true;          \
+    case 4: hipLaunchKernelGGL(K<4>, g, b, 0, st, T); return true;          \
+    case 5: hipLaunchKernelGGL(K<5>, g, b, 0, st, T); return true;          \
+    case 6: hipLaunchKernelGGL(K<6>, g, b, 0, st, T); return true;          \
+    case 7: hipLaunchKernelGGL(K<7>, g, b, 0, st, T); return true;          \
+    case 8: hipLaunchKernelGGL(K<8>, g, b, 0, st, T); return true;          \
+  }                                                                         \
+  return false;
+  switch (kind) {
+    case BK_EIG_LIN:
+      if (key == 4) hipLaunchKernelGGL(k_eig_lin_batch<4>, g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL(k_eig_lin_batch<8>, g, b, 0, st, T);
+      else if (key == 16) hipLaunchKernelGGL(k_eig_lin_batch<16>, g, b, 0, st, T);
+      else return false;
+      return true;
+    case BK_EIG_NOM:
+      if (key == 4) hipLaunchKernelGGL(k_eig_nom_batch<4>, g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL(k_eig_nom_batch<8>, g, b, 0, st, T);
+      else if (key == 16) hipLaunchKernelGGL(k_eig_nom_batch<16>, g, b, 0, st, T);
+      else return false;
+      return true;
+    case BK_COMPACT: hipLaunchKernelGGL(k_compact_batch, g, b, 0, st, T); return true;
+    case BK_FIT_FP64:
+      switch (key) {
+        case 1: hipLaunchKernelGGL(k_fit_fp64_batch<1>, g, b, 0, st, T); return true;
+        case 2: hipLaunchKernelGGL(k_fit_fp64_batch<2>, g, b, 0, st, T); return true;
+        case 3: hipLaunchKernelGGL(k_fit_fp64_batch<3>, g, b, 0, st, T); return true;
+        case 4: hipLaunchKernelGGL(k_fit_fp64_batch<4>, g, b, 0, st, T); return true;
+      }
+      return false;
+    case BK_SWEEP_FP64: CE_NS_CASES(k_sweep_fp64_batch)
+    case BK_SCAN_FP64: hipLaunchKernelGGL(k_scan_fp64_batch, g, b, 0, st, T); return true;
+  }
+  return false;
 }
 
 void launch_info_init(const float* in, double* out, size_t count, hipStream_t st) {

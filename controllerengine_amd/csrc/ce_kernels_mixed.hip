@@ -70,18 +70,18 @@ __device__ __forceinline__ Cplx geo_sum(double sig, double beta, double tau0, do
 // reverse.  Lane = list slot (coalesced reads of the SoA trajectory), 8 modes per lane
 // (blockIdx.y selects modes 8y..8y+7), blockIdx.z = time part.
 constexpr int ATB_MODES = 8;
-__global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int idx = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void k_atb_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int idx = bx * 64 + threadIdx.x;
   if (idx >= *B.count) return;
   const int c = B.list[idx];
   const double* row = B.info + (size_t)c * P->info_w;
   if (B.flags[c] & F_UNSTABLE) return;
   if (!(row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)))) return;
   const int u = B.uniq[c];
-  const int e0 = blockIdx.y * ATB_MODES;
+  const int e0 = by * ATB_MODES;
   if (e0 >= u) return;
   const int n = B.n, N = P->fit_count;
-  const int tb = blockIdx.z * B.atb_chunk;
+  const int tb = bz * B.atb_chunk;
   const int te = (tb + B.atb_chunk < N) ? tb + B.atb_chunk : N;
   const double ts = P->ts;
   const double yss = row[P->info_freqresp + 3];
@@ -113,11 +113,28 @@ __global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, St
       ai[k] = ni;
     }
   }
-  double2* out = B.atbp + ((size_t)blockIdx.z * P->nt + e0) * n + idx;
+  double2* out = B.atbp + ((size_t)bz * P->nt + e0) * n + idx;
 #pragma unroll
   for (int k = 0; k < ATB_MODES; k++)
     if (e0 + k < u) out[(size_t)k * n] = make_double2(ar[k], ai[k]);
 }
+
+__global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_atb_body(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+// local block -> (candidate block, mode block, time part) of the problem's own grid
+__global__ __launch_bounds__(64) void k_atb_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    const ProblemDev* P = T.probs[q.p];
+    const StageBufs& B = T.bufs[q.p];
+    const int nbx = (B.n + 63) / 64, nby = (P->nt + ATB_MODES - 1) / ATB_MODES;
+    const int r = q.lb / nbx;
+    k_atb_body(P, B, q.lb % nbx, nbx, r % nby, r / nby);
+  }
+}
+
 
 template <int NB>
 struct CfShape {
@@ -130,7 +147,7 @@ struct CfShape {
 // Gram matrix in closed form, right-hand side from the partial sums of k_atb, then the LDL^T
 // solve and score.  G = 16 lanes per candidate, 4 candidates per single-wave workgroup.
 template <int NB>
-__global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, StageBufs B) {
+__device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   using S = CfShape<NB>;
   constexpr int G = S::G, CPW = S::CPW, NTRI = S::NTRI, NCP = S::NCP;
   __shared__ double gram[CPW][NTRI];
@@ -138,7 +155,7 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
   __shared__ int itr[CPW][NCP];
   const int lane = threadIdx.x;
   const int slot = lane / G, g = lane % G;
-  const int idx = blockIdx.x * CPW + slot;
+  const int idx = bx * CPW + slot;
   const int n = B.n;
   const int nt = P->nt, nc2 = 2 * nt;
   bool active = false;
@@ -214,6 +231,20 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
                          vec[slot][5], itr[slot], g);
 }
 
+template <int NB>
+__global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_fitcf_body<NB>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_fitcf_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_fitcf_body<NB>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // ------------------------------------------------------------------------------------------------
 // Margin sweep, mixed chain.  One wave per candidate; lane l owns the contiguous frequency chunk
 // [l*KM, l*KM + KM), KM = ceil(F / 64) <= KMAX.  Chunk-wise scans combine in-lane sequential
@@ -242,8 +273,8 @@ __device__ __forceinline__ V wave_excl_sum(V v, int lane) {
 // band of it) are marked for FP64 refinement and appended to the dense refinement list, at most
 // RMAX per candidate (crossing points first).
 template <int NS>
-__global__ __launch_bounds__(64) void k_sweep_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x;
+__device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int c = bx;
   if (margin_skip(P, B, c)) return;
   const int F = P->F, lane = threadIdx.x;
   const int KM = (F + 63) / 64;
@@ -413,10 +444,24 @@ __global__ __launch_bounds__(64) void k_sweep_mark32(const ProblemDev* __restric
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ P, StageBufs B) {
+__global__ __launch_bounds__(64) void k_sweep_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_sweep_mark32_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_sweep_mark32_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_sweep_mark32_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
+template <int NS>
+__device__ __forceinline__ void k_refine64_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int total = *B.rcount;
   const int F = P->F;
-  for (int i = blockIdx.x * 64 + threadIdx.x; i < total; i += gridDim.x * 64) {
+  for (int i = bx * 64 + threadIdx.x; i < total; i += nbx * 64) {
     const int2 e = B.rlist[i];
     const int c = e.x, w = e.y;
     double m, raw, b;
@@ -429,11 +474,25 @@ __global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ 
   }
 }
 
+template <int NS>
+__global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_refine64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_refine64_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_refine64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // Crossing scan (src/allmargin.cu:119-180) over the float values with the FP64 refinements in
 // place, one wave per candidate: each interval's contributions are independent once the phase is
 // unwrapped (chunk-wise prefix in double); the minima and flags reduce over the wave.
-__global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x;
+__device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int c = bx;
   const int lane = threadIdx.x;
   double* row = B.info + (size_t)c * P->info_w;
   double* sc = row + P->info_scoring;
@@ -578,6 +637,18 @@ __global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P,
   sc[0] = cedm::isfinite_(total) ? total : EF;
 }
 
+__global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_scan32_body(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+__global__ __launch_bounds__(64) void k_scan32_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_scan32_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+
 // ------------------------------------------------------------------------------------------------
 bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   return launch_sim<float>(ns, nd, P, B, B.ybuf32, st);
@@ -605,6 +676,37 @@ static void launch_sweep_mixed_t(int F, const ProblemDev* P, const StageBufs& B,
   kmark("k_refine64", st);
   hipLaunchKernelGGL(k_scan32, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_scan32", st);
+}
+
+bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
+  const dim3 g(grid), b(64);
+#define CE_NS_CASES(K)                                                      \
+  switch (key) {                                                            \
+    case 1: hipLaunchKernelGGL(K<1>, g, b, 0, st, T); return true;          \
+    case 2: hipLaunchKernelGGL(K<2>, g, b, 0, st, T); return true;          \
+    case 3: hipLaunchKernelGGL(K<3>, g, b, 0, st, T); return true;          \
+    case 4: hipLaunchKernelGGL(K<4>, g, b, 0, st, T); return true;          \
+    case 5: hipLaunchKernelGGL(K<5>, g, b, 0, st, T); return true;          \
+    case 6: hipLaunchKernelGGL(K<6>, g, b, 0, st, T); return true;          \
+    case 7: hipLaunchKernelGGL(K<7>, g, b, 0, st, T); return true;          \
+    case 8: hipLaunchKernelGGL(K<8>, g, b, 0, st, T); return true;          \
+  }                                                                         \
+  return false;
+  switch (kind) {
+    case BK_ATB: hipLaunchKernelGGL(k_atb_batch, g, b, 0, st, T); return true;
+    case BK_FITCF:
+      switch (key) {
+        case 1: hipLaunchKernelGGL(k_fitcf_batch<1>, g, b, 0, st, T); return true;
+        case 2: hipLaunchKernelGGL(k_fitcf_batch<2>, g, b, 0, st, T); return true;
+        case 3: hipLaunchKernelGGL(k_fitcf_batch<3>, g, b, 0, st, T); return true;
+        case 4: hipLaunchKernelGGL(k_fitcf_batch<4>, g, b, 0, st, T); return true;
+      }
+      return false;
+    case BK_SWEEP_MARK32: CE_NS_CASES(k_sweep_mark32_batch)
+    case BK_REFINE64: CE_NS_CASES(k_refine64_batch)
+    case BK_SCAN32: hipLaunchKernelGGL(k_scan32_batch, g, b, 0, st, T); return true;
+  }
+  return false;
 }
 
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {

@@ -91,9 +91,11 @@ __device__ __forceinline__ R interp_dev(const Ins* __restrict__ ops, int count, 
 // (reference src/modalsetup.cu:52-62).  max_t fl(y_t yss) equals fl(max_t y_t * yss) for yss > 0 and
 // fl(min_t y_t * yss) for yss < 0, since rounding is monotonic.
 template <typename R, int NS, int ND>
-__global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, StageBufs B, R* __restrict__ ybuf_base) {
+__device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by,
+                                           int bz) {
   using O = RealOps<R>;
-  const int idx = blockIdx.x * 64 + threadIdx.x;
+  R* __restrict__ ybuf_base = (R*)(sizeof(R) == 8 ? (void*)B.ybuf : (void*)B.ybuf32);
+  const int idx = bx * 64 + threadIdx.x;
   if (idx >= *B.count) return;
   const int c = B.list[idx];
   const bool unstable = (B.flags[c] & F_UNSTABLE) != 0;
@@ -182,8 +184,22 @@ __global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, St
 }
 
 template <typename R, int NS, int ND>
+__global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_sim_body<R, NS, ND>(P, B, blockIdx.x, gridDim.x, 0, 0);
+}
+
+template <typename R, int NS, int ND>
+__global__ __launch_bounds__(64) void k_sim_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_sim_body<R, NS, ND>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+template <typename R, int NS, int ND>
 static void launch_sim_t(const ProblemDev* P, const StageBufs& B, R* ybuf, hipStream_t st) {
-  hipLaunchKernelGGL((k_sim<R, NS, ND>), dim3((B.n + 63) / 64), dim3(64), 0, st, P, B, ybuf);
+  (void)ybuf;
+  hipLaunchKernelGGL((k_sim<R, NS, ND>), dim3((B.n + 63) / 64), dim3(64), 0, st, P, B);
   kmark(sizeof(R) == 8 ? "k_sim_fp64" : "k_sim_f32", st);
 }
 

@@ -50,6 +50,54 @@ struct StageBufs {
   float* d_atb;
 };
 
+// Batched launches: one launch of a stage kernel covers a group of problems (same kernel
+// template instance).  Problem j of the group owns the group-local blocks [start[j], start[j+1]);
+// a workgroup maps its block to (problem, local block, the problem's block count) by binary search
+// over the prefix and runs the single-problem body.  Workgroups loop over blocks with a stride of
+// the grid size, so the grid may be smaller than the block count (persistent launch).
+struct BatchTab {
+  const int32_t* start;            // J + 1 block prefix of this group
+  const int32_t* pid;              // J: batch index of each problem of the group
+  const ProblemDev* const* probs;  // per batch problem
+  const StageBufs* bufs;           // per batch problem
+  int32_t J, total;
+};
+// Stage kernels that have a batched form (key: the template parameter of the instance).
+enum BatchKernel : int {
+  BK_EIG_LIN = 0,   // key G
+  BK_EIG_NOM,       // key G
+  BK_COMPACT,
+  BK_FIT_FP64,      // key NB
+  BK_SWEEP_FP64,    // key NS
+  BK_SCAN_FP64,
+  BK_ATB,
+  BK_FITCF,         // key NB
+  BK_SWEEP_MARK32,  // key NS
+  BK_REFINE64,      // key NS
+  BK_SCAN32,
+  BK_COUNT
+};
+bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream_t st);
+bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st);
+
+struct BatchSlot {
+  int p, lb, nb;
+};
+__device__ __forceinline__ BatchSlot batch_slot(const BatchTab& T, int b) {
+  b = __builtin_amdgcn_readfirstlane(b);
+  int lo = 0, hi = T.J - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  // workgroup-uniform: keep the slot in scalar registers
+  const int p = __builtin_amdgcn_readfirstlane(T.pid[lo]);
+  const int s0 = __builtin_amdgcn_readfirstlane(T.start[lo]);
+  const int s1 = __builtin_amdgcn_readfirstlane(T.start[lo + 1]);
+  return BatchSlot{p, b - s0, s1 - s0};
+}
+
 // Optional per-kernel timing.  Launchers call kmark(name, stream) after each stage kernel; when
 // the calling thread's engine has profiling on, an event is recorded there, so that the time
 // between two marks is the duration of the kernel named by the second one.

@@ -149,6 +149,37 @@ CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
 CE_EXPORT int ce_engine_set_profiling(ce_engine* eng, int on);
 CE_EXPORT int ce_engine_kernel_times(ce_engine* eng, char* names, int names_len, double* ms, int n_max);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batched evaluation (no counterpart in the reference, which evaluates one plant per call; its
+ * serial mode, src/main.cu:441-558, loops over rounds on the host).  A batch holds many problems,
+ * each a (config, plant, candidate count) of any dimensions, horizon and frequency count.  One
+ * ce_batch_run evaluates every problem with one grouped launch per stage kernel and template
+ * instance.  Inputs are contiguous over the problems in the order they were added: problem j's
+ * gains rows start at float offset gains_off(j) and its INFO rows at info_off(j)
+ * (ce_batch_offsets); info64_out (optional) has the INFO layout in double.
+ * ce_batch_update replaces a problem's config/plant values (same row widths) between runs; the
+ * device layout, the compiled kernels and the captured graph are kept while the dimensions,
+ * frequency count, fit window and instruction structure stay the same. */
+typedef struct ce_batch ce_batch;
+CE_EXPORT ce_batch* ce_batch_create(int device, char* err, int errlen);
+CE_EXPORT void ce_batch_destroy(ce_batch* b);
+CE_EXPORT int ce_batch_clear(ce_batch* b);
+/* Returns the problem's index, or -1 with a message in err. */
+CE_EXPORT int ce_batch_add(ce_batch* b, const ce_config* cfg, const ce_plant* plant, int n, char* err, int errlen);
+CE_EXPORT int ce_batch_update(ce_batch* b, int index, const ce_config* cfg, const ce_plant* plant, char* err,
+                              int errlen);
+CE_EXPORT int ce_batch_offsets(ce_batch* b, int index, long long* gains_off, long long* info_off);
+/* use_graph: capture the stage chain into a hipGraph and replay it (default on); autotune: choose
+ * each launch's grid size by timing whole-chain passes on the next run (results do not depend on
+ * it); profiling: record an event after every launch (disables the graph). */
+CE_EXPORT int ce_batch_set_options(ce_batch* b, int use_graph, int autotune, int profiling);
+CE_EXPORT int ce_batch_run(ce_batch* b, int precision, const float* d_gains, const float* d_info_in, float* d_info_out,
+                           double* d_info64_out, void* stream, char* err, int errlen);
+/* Number of grouped launches, whether the last run replayed a graph, whether the grids are tuned. */
+CE_EXPORT int ce_batch_info(ce_batch* b, int* launches, int* used_graph, int* tuned);
+/* As ce_engine_kernel_times, for the last run with profiling on. */
+CE_EXPORT int ce_batch_kernel_times(ce_batch* b, char* names, int names_len, double* ms, int n_max);
+
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
 CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);
