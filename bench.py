@@ -95,6 +95,69 @@ def cpu_baseline(problem, target_s=12.0, max_n=None):
                     reps, n_final, n_total, t))
 
 
+def bench_c4(args):
+    """Batched uneven Monte Carlo: all plants of a round in one batched chain (grouped persistent
+    launches, hipGraph replay, autotuned grids); every round uploads the perturbed plants."""
+    import numpy as np
+    import torch
+    from controllerengine_amd import PRECISION_FP64, PRECISION_MIXED, workloads
+    from controllerengine_amd.batch import Batch
+    t0 = time.perf_counter()
+    w = workloads.C4(n_plants=args.plants, rounds=max(args.rounds, 1))
+    gen_s = time.perf_counter() - t0
+    b = Batch(0)
+    for p in w.plants:
+        b.add(p, rnd=0)
+    d_g, d_i = b.pack([p.gains for p in w.plants], [p.info for p in w.plants])
+    d_o = torch.empty_like(d_i)
+    n = w.candidates()
+    res = {}
+    modes = [("mixed", PRECISION_MIXED), ("fp64", PRECISION_FP64)]
+    if args.precision != "auto":
+        modes = [m for m in modes if m[0] == args.precision]
+    for name, prec in modes:
+        rounds = args.rounds if name == "mixed" else max(1, min(args.rounds, args.fp64_rounds))
+        b.set_options(use_graph=True, autotune=True)
+        for r in range(args.warmup):
+            b.run_device(prec, d_g, d_i, d_o)
+        torch.cuda.synchronize()
+        b.set_options(use_graph=True, autotune=False)
+        # per-kernel times of one round (events after every grouped launch, graph off)
+        b.set_options(use_graph=False, autotune=False, profiling=True)
+        b.run_device(prec, d_g, d_i, d_o)
+        torch.cuda.synchronize()
+        kt = b.kernel_times()
+        b.set_options(use_graph=True, autotune=False, profiling=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            for j, p in enumerate(w.plants):
+                b.update(j, p, rnd=r % w.rounds)
+            b.run_device(prec, d_g, d_i, d_o)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        info = b.info()
+        res[name] = dict(value=n * rounds / dt, s_per_round=dt / rounds, rounds=rounds, used_graph=info["used_graph"],
+                         launches=info["launches"], kernel_ms={k: round(v, 3) for k, v in kt})
+    head = modes[0][0]
+    line = {
+        "metric": "candidate evals/s, mixed vs FP64 plain at matched margins; p50/p99 job latency",
+        "value": res[head]["value"], "unit": "candidate-evals/s", "n_gpus": 1, "steps": res[head]["rounds"],
+        "warmup": args.warmup, "ms_per_step": 1000 * res[head]["s_per_round"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": "c4: %d plants of 2-8 states, %d candidates per round (log-uniform 256-16384 per "
+                               "plant), T 1000-8000, F=400, serial rounds with +-10%% perturbations, one batch" %
+                               (len(w.plants), n), "precision": head},
+        "batch": {k: {kk: vv for kk, vv in v.items() if kk != "kernel_ms"} for k, v in res.items()},
+        "kernel_ms": res[head]["kernel_ms"], "generator_s": gen_s,
+    }
+    if "fp64" in res and head != "fp64":
+        line["fp64_value"] = res["fp64"]["value"]
+        line["speedup_vs_fp64"] = res[head]["value"] / res["fp64"]["value"]
+    b.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +171,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--plants", type=int, default=512, help="c4: plants")
+    ap.add_argument("--rounds", type=int, default=64, help="c4: rounds (mixed)")
+    ap.add_argument("--fp64-rounds", type=int, default=4, help="c4: rounds timed for the FP64 comparison")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -122,6 +188,10 @@ def main():
         print("bench.py measures one GPU: the candidate chain has no cross-GPU step in this build "
               "(multi-GPU sharding would split candidate ranges); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "c4":
+        bench_c4(args)
+        return
 
     import numpy as np
     import torch

@@ -100,15 +100,9 @@ class Problem:
         ns, nd = self.ns, self.nd
         r = rnd if self.d_a.shape[0] > 1 else 0
 
-        def put(dst, arr):
-            flat = np.ascontiguousarray(arr, dtype=np.float32).ravel()
-            for i, v in enumerate(flat):
-                dst[i] = float(v)
-
-        def puti(dst, arr):
-            flat = np.ascontiguousarray(arr, dtype=np.int32).ravel()
-            for i, v in enumerate(flat):
-                dst[i] = int(v)
+        def put(dst, arr, dtype=np.float32):
+            flat = np.ascontiguousarray(arr, dtype=dtype).ravel()
+            np.frombuffer(dst, dtype=dtype)[:flat.size] = flat
 
         put(p.ss_a, self.ss_a.reshape(ns, ns))
         put(p.ss_a_i, self.ss_a_i.reshape(ns, ns))
@@ -121,9 +115,9 @@ class Problem:
         put(p.d_bw, self.d_bw[r])
         put(p.d_c, self.d_c[r])
         put(p.base_params, self.base_params[r if self.base_params.shape[0] > 1 else 0])
-        puti(p.dc_info, self.dc_info)
-        puti(p.u_info, self.u_info)
-        puti(p.nl_info, self.nl_info)
+        put(p.dc_info, self.dc_info, np.int32)
+        put(p.u_info, self.u_info, np.int32)
+        put(p.nl_info, self.nl_info, np.int32)
         return p
 
     def round_rows(self, rnd, n_per_round):

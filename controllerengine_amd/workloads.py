@@ -87,7 +87,10 @@ def _assemble(name, cfg, ns, nd, A, A_I, BU, BW, C, params, dc, u, nl, K, L, see
               pad_to_threads=False):
     A32 = np.float32(A)
     gains = np.concatenate([K, L], axis=1)
-    N = np.array([dc_gain_N(A, BU, C, k) for k in K])
+    # N = 1 / (C (I - A + B K^T)^-1 B) for every candidate, as one batched solve
+    ns_ = A.shape[0]
+    M = (np.eye(ns_) - A)[None] + BU[None, :, None] * K[:, None, :]
+    N = 1.0 / (np.linalg.solve(M, np.broadcast_to(BU, (K.shape[0], ns_))[..., None])[..., 0] @ C)
     gains = np.concatenate([gains, N[:, None]], axis=1).astype(np.float32)
     n = gains.shape[0]
     if pad_to_threads:
@@ -230,6 +233,106 @@ def c3(n=262144, seed=2, pad_to_threads=False, T=4000, F=512):
                **SPEC)
     return _assemble("c3", cfg, ns, ns, A, A_I, BU, BW, C, P, dc, u, nl, K, L, dict(seed=seed, K0=K0, L0=L0),
                      pad_to_threads=pad_to_threads)
+
+
+def chain_drive(masses, springs, dampers, c_motor, dt):
+    """Chain of len(masses) masses (states x1, v1, x2, v2, ...): spring/damper i joins mass i and
+    i+1, motor damping on mass 1, force input on mass 1, output x1; forward Euler at dt.
+    Returns Ac, A, A_I, BU, BW (BW: Coulomb friction on v1, drag on the last mass's velocity)."""
+    m = len(masses)
+    ns = 2 * m
+    Ac = np.zeros((ns, ns))
+    for i in range(m):
+        Ac[2 * i, 2 * i + 1] = 1.0
+    Ac[1, 1] -= c_motor / masses[0]
+    for i in range(m - 1):
+        k, c = springs[i], dampers[i]
+        a, b = i, i + 1
+        for (p_, q_, sgn) in ((a, b, 1.0), (b, a, 1.0)):
+            Ac[2 * p_ + 1, 2 * p_] -= sgn * k / masses[p_]
+            Ac[2 * p_ + 1, 2 * q_] += sgn * k / masses[p_]
+            Ac[2 * p_ + 1, 2 * p_ + 1] -= sgn * c / masses[p_]
+            Ac[2 * p_ + 1, 2 * q_ + 1] += sgn * c / masses[p_]
+    Bc = np.zeros(ns)
+    Bc[1] = 1.0 / masses[0]
+    BWc = np.zeros((ns, 2))
+    BWc[1, 0] = -1.0 / masses[0]
+    BWc[2 * m - 1, 1] = -1.0 / masses[-1]
+    A_I = dt * Ac
+    return Ac, np.eye(ns) + A_I, A_I, dt * Bc, dt * BWc
+
+
+class C4:
+    """Batched uneven Monte Carlo: n_plants flexible-drive chains of 1-4 masses (2, 4, 6 or 8
+    states, as the two-mass drive of c2: Coulomb friction on the motor mass and quadratic drag on
+    the load mass as NL chains), masses / springs / dampers drawn around the c2 values.  Each plant
+    is one serial-mode problem over `rounds` rounds: every round perturbs the physical parameters
+    and the NL constants (BASE_PARAMS) by a uniform +-10 % and re-discretises the plant
+    (DISCRETE_*); the observer model stays nominal.  Candidates per plant log-uniform in
+    [nmin, nmax] around LQR / LQR-dual seeds (as c2), horizons uniform in [tmin, tmax] steps,
+    dt = 5e-4, F = 400."""
+
+    def __init__(self, seed=3, n_plants=512, rounds=64, nmin=256, nmax=16384, tmin=1000, tmax=8000, F=400,
+                 dt=5e-4):
+        rng = np.random.default_rng(seed)
+        self.rounds = rounds
+        self.plants = []
+        for j in range(n_plants):
+            m = int(rng.integers(1, 5))
+            masses = np.concatenate([[1.0], rng.uniform(0.3, 1.0, m - 1)])
+            springs = rng.uniform(30.0, 70.0, m - 1)
+            dampers = rng.uniform(0.1, 0.3, m - 1)
+            c_motor = 0.05 if m > 1 else 0.4
+            n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+            T = int(rng.integers(tmin, tmax + 1))
+            # nominal model: the observer, and the seeds of the candidate gains
+            Ac, A, A_I, BU, BW = chain_drive(masses, springs, dampers, c_motor, dt)
+            ns = 2 * m
+            C = np.zeros(ns)
+            C[0] = 1.0
+            P = Params()
+            one, cc, cd = P.const(1.0), P.const(0.3), P.const(0.02)
+            u = [row(ns, 0, OP_MUL, PREV, one)]
+            dc = [row(ns, 0, OP_MUL, STATE, one)]
+            vl = 2 * m - 1
+            nl = [row(ns + 1, 1, OP_SGN, STATE, STATE), row(ns + 1, 0, OP_MUL, PREV, cc),
+                  row(ns + 1, vl, OP_SGN, STATE, STATE), row(ns + 1, vl, OP_MUL, PREV, STATE),
+                  row(ns + 1, vl, OP_MUL, PREV, STATE), row(ns + 1, 0, OP_MUL, PREV, cd)]
+            K0 = dlqr(A, BU, np.eye(ns), 1.0)
+            L0 = kalman_predictor(A, C, np.eye(ns), 1.0)
+            K = around(rng, K0, n)
+            L = around(rng, L0, n)
+            cfg = dict(statespace_dim=ns, discrete_dim=ns, nonlin_dim=2, n_params=len(P.vals), input_size=0,
+                       dc_instructions=1, u_instructions=1, nl_instructions=[2, 4], n_cores=128, n_rounds=rounds,
+                       mode="serial", time=T, duration=T * dt, ref_step_time=0.02, ref_amplitude=1.0,
+                       fit_start=0.02, fit_end=0.75, steady_start=0.75, bandwidth=1.0, freq=F, freq_min=1e-2,
+                       dump_diagnostics=False, **SPEC)
+            prob = _assemble("c4_%d" % j, cfg, ns, ns, A, A_I, BU, BW, C, P, dc, u, nl, K, L,
+                             dict(seed=seed, plant=j))
+            # per-round perturbed plants (serial-mode slices) and BASE_PARAMS
+            da, dai, dbu, dbw, dcs, bps = [], [], [], [], [], []
+            base = P.array()
+            for r in range(rounds):
+                f = 1.0 + rng.uniform(-0.1, 0.1, 2 * m + 2)
+                _, Ar, Ar_I, BUr, BWr = chain_drive(masses * f[:m], springs * f[m:2 * m - 1] if m > 1 else springs,
+                                                     dampers * f[m:2 * m - 1] if m > 1 else dampers,
+                                                     c_motor * f[2 * m - 1], dt)
+                da.append(np.float32(Ar))
+                dai.append(np.float32(Ar_I))
+                dbu.append(np.float32(BUr))
+                dbw.append(np.float32(BWr))
+                dcs.append(np.float32(C))
+                bp = base.copy()
+                bp[1] = np.float32(0.3 * f[2 * m])
+                bp[2] = np.float32(0.02 * f[2 * m + 1])
+                bps.append(bp)
+            prob.d_a, prob.d_a_i = np.stack(da), np.stack(dai)
+            prob.d_bu, prob.d_bw, prob.d_c = np.stack(dbu), np.stack(dbw), np.stack(dcs)
+            prob.base_params = np.stack(bps)
+            self.plants.append(prob)
+
+    def candidates(self):
+        return sum(p.gains.shape[0] for p in self.plants)
 
 
 CONFIGS = {"c1": c1, "c2": c2, "c3": c3}
