@@ -158,6 +158,71 @@ def bench_c4(args):
     print(json.dumps(line))
 
 
+def bench_c5(args):
+    """Serving loop: Poisson arrivals of c5 jobs at each rate for `duration` seconds, submitted to a
+    ce_server (footprint admission, continuous batching); p50 / p99 latency and throughput."""
+    import numpy as np
+    from controllerengine_amd import PRECISION_FP64, PRECISION_MIXED, workloads
+    from controllerengine_amd.server import Server
+    rng = np.random.default_rng(4)
+    pool = [workloads.c5_job(rng) for _ in range(args.pool)]
+    rates = [float(r) for r in args.rates.split(",")]
+    modes = [("mixed", PRECISION_MIXED), ("fp64", PRECISION_FP64)]
+    if args.precision != "auto":
+        modes = [m for m in modes if m[0] == args.precision]
+    out = {}
+    for name, prec in modes:
+        srv = Server(prec, slots=args.slots)
+        for j in pool:  # compiles each simulation structure once, outside the measurement
+            srv.submit(j)
+        srv.drain(600.0)
+        out[name] = {}
+        for rate in rates:
+            srv.reset_stats()
+            arr = np.random.default_rng(int(rate) * 7 + 4)
+            t0 = time.perf_counter()
+            t_next = 0.0
+            ids = []
+            cands = 0
+            while True:
+                t_next += arr.exponential(1.0 / rate)
+                if t_next > args.duration:
+                    break
+                now = time.perf_counter() - t0
+                if t_next > now:
+                    time.sleep(t_next - now)
+                j = pool[int(arr.integers(len(pool)))]
+                ids.append(srv.submit(j))
+                cands += j.gains.shape[0]
+            srv.drain(600.0)
+            st = srv.stats()
+            for jid in ids:
+                srv.poll(jid)
+            out[name][str(int(rate))] = dict(jobs=int(st["jobs"]), p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
+                                             mean_ms=st["mean_ms"], max_ms=st["max_ms"],
+                                             evals_per_s=st["evals_per_s"], offered_evals_per_s=cands / args.duration,
+                                             batches=int(st["batches"]),
+                                             peak_jobs_in_flight=int(st["peak_jobs_in_flight"]),
+                                             peak_batches_in_flight=int(st["peak_batches_in_flight"]))
+            print("rate %g %s: %s" % (rate, name, out[name][str(int(rate))]), file=sys.stderr, flush=True)
+        srv.close()
+    head = modes[0][0]
+    top = str(int(max(rates)))
+    line = {
+        "metric": "candidate evals/s, mixed vs FP64 plain at matched margins; p50/p99 job latency",
+        "value": out[head][top]["evals_per_s"], "unit": "candidate-evals/s", "n_gpus": 1,
+        "steps": out[head][top]["jobs"], "warmup": len(pool), "ms_per_step": out[head][top]["mean_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": "c5: Poisson job arrivals at %s jobs/s for %g s each; jobs: random stable plants "
+                               "of 2-8 states, 0-4 NL, 256-65536 candidates, T 500-5000, F 128-512 (pool of %d)"
+                               % (args.rates, args.duration, len(pool)), "precision": head},
+        "serving": out,
+        "p50_ms": out[head][top]["p50_ms"], "p99_ms": out[head][top]["p99_ms"],
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +239,10 @@ def main():
     ap.add_argument("--plants", type=int, default=512, help="c4: plants")
     ap.add_argument("--rounds", type=int, default=64, help="c4: rounds (mixed)")
     ap.add_argument("--fp64-rounds", type=int, default=4, help="c4: rounds timed for the FP64 comparison")
+    ap.add_argument("--rates", default="20,50,100", help="c5: arrival rates (jobs/s)")
+    ap.add_argument("--duration", type=float, default=10.0, help="c5: seconds of arrivals per rate")
+    ap.add_argument("--pool", type=int, default=64, help="c5: distinct generated jobs")
+    ap.add_argument("--slots", type=int, default=2, help="c5: batches in flight")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -191,6 +260,9 @@ def main():
 
     if args.config == "c4":
         bench_c4(args)
+        return
+    if args.config == "c5":
+        bench_c5(args)
         return
 
     import numpy as np

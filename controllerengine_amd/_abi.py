@@ -71,6 +71,8 @@ EXPORTS = [
     "ce_engine_last_sim_jit", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_detmath_host",
     "ce_detmath_device", "ce_batch_create", "ce_batch_destroy", "ce_batch_clear", "ce_batch_add", "ce_batch_update",
     "ce_batch_offsets", "ce_batch_set_options", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
+    "ce_server_create", "ce_server_destroy", "ce_server_submit", "ce_server_poll", "ce_server_drain",
+    "ce_server_stats", "ce_server_reset_stats",
 ]
 
 _lib = None
@@ -128,6 +130,21 @@ def _declare(lib):
     lib.ce_batch_info.restype = ctypes.c_int
     lib.ce_batch_kernel_times.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
     lib.ce_batch_kernel_times.restype = ctypes.c_int
+    lib.ce_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_char_p, ctypes.c_int]
+    lib.ce_server_create.restype = vp
+    lib.ce_server_destroy.argtypes = [vp]
+    lib.ce_server_destroy.restype = None
+    lib.ce_server_submit.argtypes = [vp, P(CeConfig), P(CePlant), vp, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_server_submit.restype = ctypes.c_longlong
+    lib.ce_server_poll.argtypes = [vp, ctypes.c_longlong, vp, vp, ctypes.c_char_p, ctypes.c_int]
+    lib.ce_server_poll.restype = ctypes.c_int
+    lib.ce_server_drain.argtypes = [vp, ctypes.c_double]
+    lib.ce_server_drain.restype = ctypes.c_int
+    lib.ce_server_stats.argtypes = [vp, P(ctypes.c_double), ctypes.c_int]
+    lib.ce_server_stats.restype = ctypes.c_int
+    lib.ce_server_reset_stats.argtypes = [vp]
+    lib.ce_server_reset_stats.restype = ctypes.c_int
     return lib
 
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -136,7 +137,6 @@ struct ce_batch {
   std::vector<hipEvent_t> mark_ev;
   std::vector<std::string> mark_name;
   int n_marks = 0;
-  JitCache* jit = nullptr;
   int last_used_graph = 0;
 };
 
@@ -198,6 +198,17 @@ static void build_blobs(ce_batch* b, int precision) {
 }
 
 static int g_eig(int m) { return m <= 4 ? 4 : (m <= 8 ? 8 : 16); }
+
+// Compiled simulation kernels are shared by every batch of the process on the same device (the
+// caches are thread-safe); modules stay loaded until the process ends.
+static JitCache* shared_jit(int device) {
+  static std::mutex mu;
+  static std::map<int, JitCache*> caches;
+  std::lock_guard<std::mutex> lk(mu);
+  JitCache*& c = caches[device];
+  if (!c) c = jit_cache_create();
+  return c;
+}
 
 // Device layout of all buffers, StageBufs, launch groups and their tables.
 static void build_layout(ce_batch* b, int precision, hipStream_t st) {
@@ -350,9 +361,8 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
         L.kind = SIM_LAUNCH;
         L.skey = k;
         L.start.push_back(0);
-        if (!b->jit) b->jit = jit_cache_create();
         std::string e;
-        if (!jit_get(b->jit, gen_sim_batch_source(H, mixed), &L.fn, e)) throw std::runtime_error(e);
+        if (!jit_get(shared_jit(b->device), gen_sim_batch_source(H, mixed), &L.fn, e)) throw std::runtime_error(e);
       }
       L.pid.push_back(j);
       L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64));
@@ -611,7 +621,6 @@ extern "C" CE_EXPORT void ce_batch_destroy(ce_batch* b) {
   if (b->ev_in) hipEventDestroy(b->ev_in);
   if (b->ev_out) hipEventDestroy(b->ev_out);
   if (b->own) hipStreamDestroy(b->own);
-  jit_cache_destroy(b->jit);
   delete b;
 }
 

@@ -335,4 +335,60 @@ class C4:
         return sum(p.gains.shape[0] for p in self.plants)
 
 
+def c5_job(rng, nmin=256, nmax=65536, tmin=500, tmax=5000, fmin=128, fmax=512, n=None):
+    """One serving job: a random stable plant as in c3 with dimension in 2..8 (conjugate pairs plus
+    one real pole for odd dimensions), 0-4 nonlinear terms chosen from saturation, sign, product
+    and x/(1+|x|), candidates log-uniform in [nmin, nmax] around LQR / Kalman seeds, horizon
+    uniform in [tmin, tmax] steps at dt = 1e-3, F uniform in [fmin, fmax]."""
+    from scipy.linalg import expm
+    ns = int(rng.integers(2, 9))
+    dt = 1e-3
+    J = np.zeros((ns, ns))
+    for p in range(ns // 2):
+        sr, w = rng.uniform(-20.0, -0.5), rng.uniform(0.0, 30.0)
+        J[2 * p:2 * p + 2, 2 * p:2 * p + 2] = [[sr, w], [-w, sr]]
+    if ns % 2:
+        J[ns - 1, ns - 1] = rng.uniform(-20.0, -0.5)
+    V, _ = np.linalg.qr(rng.standard_normal((ns, ns)))
+    Ac = V @ J @ V.T
+    Bc = rng.standard_normal(ns)
+    M = np.zeros((ns + 1, ns + 1))
+    M[:ns, :ns] = Ac * dt
+    M[:ns, ns] = Bc * dt
+    E = expm(M)
+    A, BU = E[:ns, :ns], E[:ns, ns]
+    A_I = A - np.eye(ns)
+    C = np.zeros(ns)
+    C[0] = 1.0
+    nw = int(rng.integers(0, 5))
+    P = Params()
+    one, zero, mone = P.const(1.0), P.const(0.0), P.const(-1.0)
+    umin, umax = P.const(-50.0), P.const(50.0)
+    u = [row(ns, 0, OP_SAT, umin, umax)]
+    dc = [row(ns, 0, OP_MUL, STATE, one)]
+    w = ns + 1
+    last = ns - 1
+    chains = [[row(w, 0, OP_SAT, mone, one)],                                          # clamp(x0, -1, 1)
+              [row(w, min(1, last), OP_SGN, STATE, STATE)],                            # sign(x1)
+              [row(w, 0, OP_ADD, STATE, zero), row(w, last, OP_MUL, PREV, STATE)],     # x0 * x_last
+              [row(w, last, OP_SGN, STATE, STATE), row(w, last, OP_MUL, PREV, STATE),  # x/(1+|x|)
+               row(w, 0, OP_ADD, PREV, one), row(w, last, OP_DIV, STATE, PREV)]]
+    pick = sorted(rng.choice(4, size=nw, replace=False)) if nw else []
+    nl = [r for q in pick for r in chains[q]]
+    BW = dt * 0.5 * rng.standard_normal((ns, nw))
+    K0 = dlqr(A, BU, np.eye(ns), 1.0)
+    L0 = kalman_predictor(A, C, np.eye(ns), 0.01)
+    n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))) if n is None else n
+    K = around(rng, K0, n)
+    L = around(rng, L0, n)
+    T = int(rng.integers(tmin, tmax + 1))
+    F = int(rng.integers(fmin, fmax + 1))
+    cfg = dict(statespace_dim=ns, discrete_dim=ns, nonlin_dim=nw, n_params=len(P.vals), input_size=0,
+               dc_instructions=1, u_instructions=1, nl_instructions=[len(chains[q]) for q in pick], n_cores=128,
+               n_rounds=1, mode="batch", time=T, duration=T * dt, ref_step_time=0.025, ref_amplitude=1.0,
+               fit_start=0.025, fit_end=0.75, steady_start=0.75, bandwidth=2.0, freq=F, freq_min=1e-2,
+               dump_diagnostics=False, **SPEC)
+    return _assemble("c5", cfg, ns, ns, A, A_I, BU, BW, C, P, dc, u, nl, K, L, dict(ns=ns, nw=nw))
+
+
 CONFIGS = {"c1": c1, "c2": c2, "c3": c3}

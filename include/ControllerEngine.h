@@ -180,6 +180,29 @@ CE_EXPORT int ce_batch_info(ce_batch* b, int* launches, int* used_graph, int* tu
 /* As ce_engine_kernel_times, for the last run with profiling on. */
 CE_EXPORT int ce_batch_kernel_times(ce_batch* b, char* names, int names_len, double* ms, int n_max);
 
+/* ---------------------------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference).  Jobs (config, plant, candidate rows on the host)
+ * are submitted from any thread; a scheduler thread admits queued jobs in arrival order while
+ * their estimated device footprint fits in mem_budget_bytes (<= 0: half the free device memory)
+ * minus the jobs in flight, forms a batch of up to max_jobs_per_batch jobs on one of `slots`
+ * streams (batches in flight overlap on the device) and copies the rows back.  A job's rows do not
+ * depend on the other jobs of its batch.  precision applies to every job of the server. */
+typedef struct ce_server ce_server;
+CE_EXPORT ce_server* ce_server_create(int device, int precision, long long mem_budget_bytes, int slots,
+                                      int max_jobs_per_batch, char* err, int errlen);
+CE_EXPORT void ce_server_destroy(ce_server* s);
+/* Returns the job id (>= 0) or -1.  gains: n x (2 statespace_dim + 1), info: n x info width. */
+CE_EXPORT long long ce_server_submit(ce_server* s, const ce_config* cfg, const ce_plant* plant, const float* h_gains,
+                                     const float* h_info, int n, char* err, int errlen);
+/* 0 pending, 1 done (rows copied to the non-NULL outputs, job forgotten), -1 unknown, -2 failed. */
+CE_EXPORT int ce_server_poll(ce_server* s, long long job, float* h_info_out, double* h_info64_out, char* err,
+                             int errlen);
+CE_EXPORT int ce_server_drain(ce_server* s, double timeout_s);
+/* jobs done, latency p50 / p99 / mean / max (ms), candidate-evals/s (first submission to last
+ * completion), batches, peak jobs in flight, peak batches in flight, peak bytes admitted, budget. */
+CE_EXPORT int ce_server_stats(ce_server* s, double* out, int n_max);
+CE_EXPORT int ce_server_reset_stats(ce_server* s);
+
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
 CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);
