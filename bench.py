@@ -354,6 +354,14 @@ def main():
                      "unit": "TFLOP/s", "frac": (achieved / peak) if achieved is not None else None,
                      "traffic": traffic, "flops_per_launch": fl.get(dom)},
     }
+    # the same workload through host buffers (H2D of gains and INFO rows, D2H of the rows)
+    prec_head = dict(modes)[head]
+    eng.evaluate(problem, precision=prec_head)
+    t0 = time.perf_counter()
+    hs = max(1, min(args.steps, 3))
+    for _ in range(hs):
+        eng.evaluate(problem, precision=prec_head)
+    line["host_buffers_value"] = n * hs / (time.perf_counter() - t0)
     if "fp64" in results and head != "fp64":
         line["fp64_value"] = results["fp64"]["value"]
         line["speedup_vs_fp64"] = r["value"] / results["fp64"]["value"]
