@@ -287,6 +287,8 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
   // the sweep points go to memory and are read back below, so that nothing else is live across
   // the register-heavy elimination
+  float H[NS][NS], kq[NS], lq[NS];
+  hess_prep<NS>(oa, gg, H, kq, lq);
 #pragma unroll 1
   for (int k = 0; k < nk; k++) {
     const int w = w0 + k;
@@ -295,7 +297,7 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
       failp = true;
     } else {
       float m1, r1, bq;
-      if (sweep_point<NS, float>(P, oa, gg, w, m1, r1, bq)) { mm = m1; rr = r1; b2 = bq; }
+      if (hess_point<NS>(P, H, kq, lq, w, m1, r1, bq)) { mm = m1; rr = r1; b2 = bq; }
     }
     mr[3 * w] = mm;
     mr[3 * w + 1] = rr;
