@@ -169,15 +169,43 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   o << "    const " << T << " u_raw = " << g.fma() << "(N, rr, -kx);\n";
   const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", ins_cst(P.u_ops, CE_OFF(u_ops), CE_OFF(fu_aval), CE_OFF(fu_bval)));
   o << "    const " << T << " uu = " << u << ";\n";
-  o << "    " << T << " cx = " << g.zero() << ";\n";
-  for (int k = 0; k < ns; k++)
-    o << "    cx = " << g.fma() << "(" << K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4) << ", xh" << k << ", cx);\n";
-  o << "    const " << T << " yerr = y - " << g.fma() << "(" << K(P.sd, CE_OFF(sd), CE_OFF(fsd)) << ", uu, cx);\n";
-  for (int i = 0; i < ns; i++) {
-    o << "    " << T << " dh" << i << " = " << g.fma() << "(L" << i << ", yerr, " << K(P.sb[i], CE_OFF(sb) + i * 8, CE_OFF(fsb) + i * 4) << " * uu);\n";
+  if (!mixed) {
+    // FP64: every term, in the interpreted kernel's order (bit parity with the oracle)
+    o << "    " << T << " cx = " << g.zero() << ";\n";
     for (int k = 0; k < ns; k++)
-      o << "    dh" << i << " = " << g.fma() << "(" << K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4)
-        << ", xh" << k << ", dh" << i << ");\n";
+      o << "    cx = " << g.fma() << "(" << K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4) << ", xh" << k << ", cx);\n";
+    o << "    const " << T << " yerr = y - " << g.fma() << "(" << K(P.sd, CE_OFF(sd), CE_OFF(fsd)) << ", uu, cx);\n";
+    for (int i = 0; i < ns; i++) {
+      o << "    " << T << " dh" << i << " = " << g.fma() << "(L" << i << ", yerr, " << K(P.sb[i], CE_OFF(sb) + i * 8, CE_OFF(fsb) + i * 4) << " * uu);\n";
+      for (int k = 0; k < ns; k++)
+        o << "    dh" << i << " = " << g.fma() << "(" << K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4)
+          << ", xh" << k << ", dh" << i << ");\n";
+    }
+  } else {
+    // float: terms whose coefficient is zero in float are left out (fma(0, x, a) = a for finite x;
+    // the zero pattern is part of the structure key of the batched kernels)
+    auto acc = [&](const std::string& var, const std::vector<std::pair<std::string, std::string>>& terms,
+                   const std::string& init) {
+      o << "    " << T << " " << var << " = " << init << ";\n";
+      for (auto& t : terms) o << "    " << var << " = " << g.fma() << "(" << t.first << ", " << t.second << ", " << var << ");\n";
+    };
+    std::vector<std::pair<std::string, std::string>> tc;
+    for (int k = 0; k < ns; k++)
+      if ((float)P.sc[k] != 0.0f) tc.push_back({K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4), "xh" + std::to_string(k)});
+    if ((float)P.sd != 0.0f) tc.push_back({K(P.sd, CE_OFF(sd), CE_OFF(fsd)), "uu"});
+    acc("cx", tc, g.zero());
+    o << "    const " << T << " yerr = y - cx;\n";
+    for (int i = 0; i < ns; i++) {
+      // fma(L, yerr, sb u) as in the full form; with sb = 0 it is the rounded product L yerr
+      const std::string init = (float)P.sb[i] != 0.0f
+          ? g.fma() + "(L" + std::to_string(i) + ", yerr, " + K(P.sb[i], CE_OFF(sb) + i * 8, CE_OFF(fsb) + i * 4) + " * uu)"
+          : "L" + std::to_string(i) + " * yerr";
+      std::vector<std::pair<std::string, std::string>> td;
+      for (int k = 0; k < ns; k++)
+        if ((float)P.sai[i * ns + k] != 0.0f)
+          td.push_back({K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4), "xh" + std::to_string(k)});
+      acc("dh" + std::to_string(i), td, init);
+    }
   }
   for (int i = 0; i < ns; i++) o << "    xh" << i << " = xh" << i << " + dh" << i << ";\n";
   snl = sxp;
@@ -190,13 +218,23 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
     W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, "", ins_cst(P.nl_ops[q], oi, ofa, ofb)));
   }
   for (int i = 0; i < nd; i++) {
-    o << "    " << T << " ph" << i << " = " << K(P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4) << " * uu;\n";
-    for (int q = 0; q < nw; q++)
-      o << "    ph" << i << " = " << g.fma() << "(" << K(P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4)
-        << ", " << W[q] << ", ph" << i << ");\n";
-    for (int k = 0; k < nd; k++)
-      o << "    ph" << i << " = " << g.fma() << "(" << K(P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4)
-        << ", xp" << k << ", ph" << i << ");\n";
+    if (!mixed) {
+      o << "    " << T << " ph" << i << " = " << K(P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4) << " * uu;\n";
+      for (int q = 0; q < nw; q++)
+        o << "    ph" << i << " = " << g.fma() << "(" << K(P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4)
+          << ", " << W[q] << ", ph" << i << ");\n";
+      for (int k = 0; k < nd; k++)
+        o << "    ph" << i << " = " << g.fma() << "(" << K(P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4)
+          << ", xp" << k << ", ph" << i << ");\n";
+    } else {
+      o << "    " << T << " ph" << i << " = " << g.zero() << ";\n";
+      auto term = [&](double v, size_t od, size_t of, const std::string& x) {
+        if ((float)v != 0.0f) o << "    ph" << i << " = " << g.fma() << "(" << K(v, od, of) << ", " << x << ", ph" << i << ");\n";
+      };
+      term(P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4, "uu");
+      for (int q = 0; q < nw; q++) term(P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4, W[q]);
+      for (int k = 0; k < nd; k++) term(P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4, "xp" + std::to_string(k));
+    }
   }
   for (int i = 0; i < nd; i++) o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
   o << "  }\n";
@@ -358,6 +396,19 @@ std::string sim_structure_key(const ProblemDev& P, bool mixed) {
   chain(P.dc_ops, P.dc_n);
   chain(P.u_ops, P.u_n);
   for (int q = 0; q < P.nw; q++) chain(P.nl_ops[q], P.nl_n[q]);
+  if (mixed) {  // the float kernels leave out zero coefficients
+    auto zp = [&](const double* v, int n) {
+      for (int i = 0; i < n; i++) k << ((float)v[i] != 0.0f ? '1' : '0');
+      k << "|";
+    };
+    zp(P.sc, P.ns);
+    zp(&P.sd, 1);
+    zp(P.sb, P.ns);
+    zp(P.sai, P.ns * P.ns);
+    zp(P.dbu, P.nd);
+    zp(P.dbw, P.nd * P.nw);
+    zp(P.dai, P.nd * P.nd);
+  }
   return k.str();
 }
 
