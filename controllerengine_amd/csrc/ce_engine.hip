@@ -19,6 +19,10 @@
 namespace ce {
 // kernels (ce_kernels_fp64.hip, ce_kernels_mixed.hip)
 void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_scan32(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
@@ -73,7 +77,11 @@ struct ce_engine {
   bool events_ok = false;
   hipStream_t last_stream = nullptr;
   JitCache* jit = nullptr;
-  int last_sim_jit = 0;  // 1 when the last simulation ran the specialised kernel
+  int last_sim_jit = 0;
+  // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
+  // simulation and the fit (forked after the LIN_A eigensolve, joined before the crossing scan)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // 1 when the last simulation ran the specialised kernel
 };
 
 static void* ensure(ce_engine::Buf& b, size_t bytes) {
@@ -217,7 +225,23 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
   MarkerScope marks(E, st);
   HIPCHK(hipEventRecord(E->ev[0], st));
-  launch_eig_fp64(H.ns, H.nt, P, B, st);
+  // with per-kernel timing on, everything stays on one stream so that the events bracket kernels
+  const bool fork = mixed && !E->profiling;
+  if (fork && !E->side) {
+    HIPCHK(hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
+  }
+  if (fork) {
+    launch_eig_lin(H.nt, P, B, st);
+    HIPCHK(hipEventRecord(E->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
+    launch_eig_nom(H.ns, P, B, E->side);
+    if (!launch_mark_refine(ns, F, P, B, E->side)) throw std::runtime_error("no sweep kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev_join, E->side));
+  } else {
+    launch_eig_fp64(H.ns, H.nt, P, B, st);
+  }
   launch_compact(P, B, st);
   HIPCHK(hipEventRecord(E->ev[1], st));
   // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
@@ -246,8 +270,12 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     throw std::runtime_error("no fit kernel for these dimensions");
   HIPCHK(hipEventRecord(E->ev[3], st));
   // stage 4: margin sweep, crossings and total score
-  if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st)))
+  if (fork) {
+    HIPCHK(hipStreamWaitEvent(st, E->ev_join, 0));
+    launch_scan32(P, B, st);
+  } else if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st))) {
     throw std::runtime_error("no sweep kernel for these dimensions");
+  }
   HIPCHK(hipEventRecord(E->ev[4], st));
   HIPCHK(hipGetLastError());
   launch_info_final(B.info, d_info_out, nn * H.info_w, st);
@@ -286,6 +314,9 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (E->blob_pinned) hipHostFree(E->blob_pinned);
   jit_cache_destroy(E->jit);
   for (hipEvent_t ev : E->mark_ev) hipEventDestroy(ev);
+  if (E->ev_fork) hipEventDestroy(E->ev_fork);
+  if (E->ev_join) hipEventDestroy(E->ev_join);
+  if (E->side) hipStreamDestroy(E->side);
   if (E->events_ok)
     for (int i = 0; i < N_EVENTS; i++) hipEventDestroy(E->ev[i]);
   delete E;

@@ -559,16 +559,24 @@ bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hi
   return launch_sim<double>(ns, nd, P, B, B.ybuf, st);
 }
 
-void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   if (nt <= 4) hipLaunchKernelGGL(k_eig_lin<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
   else if (nt <= 8) hipLaunchKernelGGL(k_eig_lin<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL(k_eig_lin<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
   kmark("k_eig_lin", st);
+}
+
+void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int m = 2 * ns;
   if (m <= 4) hipLaunchKernelGGL(k_eig_nom<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
   else if (m <= 8) hipLaunchKernelGGL(k_eig_nom<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL(k_eig_nom<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
   kmark("k_eig_nom", st);
+}
+
+void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  launch_eig_lin(nt, P, B, st);
+  launch_eig_nom(ns, P, B, st);
 }
 
 void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st) {

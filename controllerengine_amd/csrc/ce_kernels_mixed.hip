@@ -275,7 +275,7 @@ __device__ __forceinline__ V wave_excl_sum(V v, int lane) {
 template <int NS>
 __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int c = bx;
-  if (margin_skip(P, B, c)) return;
+  if (margin_skip_flags(B, c)) return;
   const int F = P->F, lane = threadIdx.x;
   const int KM = (F + 63) / 64;
   const int w0 = lane * KM;
@@ -671,11 +671,31 @@ bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream
 }
 
 template <int NS>
-static void launch_sweep_mixed_t(int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+static void launch_mark_refine_t(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_sweep_mark32<NS>, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_sweep_mark32", st);
   hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
   kmark("k_refine64", st);
+}
+
+// float sweep + marking + FP64 refinement (independent of the simulation and the fit)
+bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  if (F > 64 * KMAX) return false;
+  switch (ns) {
+    case 1: launch_mark_refine_t<1>(P, B, st); return true;
+    case 2: launch_mark_refine_t<2>(P, B, st); return true;
+    case 3: launch_mark_refine_t<3>(P, B, st); return true;
+    case 4: launch_mark_refine_t<4>(P, B, st); return true;
+    case 5: launch_mark_refine_t<5>(P, B, st); return true;
+    case 6: launch_mark_refine_t<6>(P, B, st); return true;
+    case 7: launch_mark_refine_t<7>(P, B, st); return true;
+    case 8: launch_mark_refine_t<8>(P, B, st); return true;
+  }
+  return false;
+}
+
+// crossing scan and total score (after the fit and the refinement)
+void launch_scan32(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_scan32, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_scan32", st);
 }
@@ -712,18 +732,9 @@ bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStrea
 }
 
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  if (F > 64 * KMAX) return false;
-  switch (ns) {
-    case 1: launch_sweep_mixed_t<1>(F, P, B, st); return true;
-    case 2: launch_sweep_mixed_t<2>(F, P, B, st); return true;
-    case 3: launch_sweep_mixed_t<3>(F, P, B, st); return true;
-    case 4: launch_sweep_mixed_t<4>(F, P, B, st); return true;
-    case 5: launch_sweep_mixed_t<5>(F, P, B, st); return true;
-    case 6: launch_sweep_mixed_t<6>(F, P, B, st); return true;
-    case 7: launch_sweep_mixed_t<7>(F, P, B, st); return true;
-    case 8: launch_sweep_mixed_t<8>(F, P, B, st); return true;
-  }
-  return false;
+  if (!launch_mark_refine(ns, F, P, B, st)) return false;
+  launch_scan32(P, B, st);
+  return true;
 }
 
 }  // namespace ce

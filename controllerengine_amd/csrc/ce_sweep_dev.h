@@ -17,6 +17,12 @@ __device__ __forceinline__ const Tp* tab(const ProblemDev* P, int64_t off) {
   return reinterpret_cast<const Tp*>(reinterpret_cast<const uint8_t*>(P) + off);
 }
 
+// Flags-only form, for sweeps that run beside the modal fit: the score the fit writes may still
+// turn the candidate into a skipped one, which the crossing scan decides after both.
+__device__ __forceinline__ bool margin_skip_flags(const StageBufs& B, int c) {
+  return (B.flags[c] & (F_NOMINAL_SKIPPED | F_NOMINAL_UNSTABLE)) != 0;
+}
+
 __device__ __forceinline__ bool margin_skip(const ProblemDev* P, const StageBufs& B, int c) {
   const int f = B.flags[c];
   if (f & (F_NOMINAL_SKIPPED | F_NOMINAL_UNSTABLE)) return true;

@@ -90,6 +90,8 @@ def test_exports_match_header():
     lib = _abi.lib()
     for name in declared:
         assert hasattr(lib, name), name
+    # every symbol the library needs resolves at load time (no missing internal definitions)
+    ctypes.CDLL(_abi.LIB_PATH, mode=os.RTLD_NOW | os.RTLD_LOCAL)
 
 
 # ------------------------------------------------------------------------------------------ GPU
