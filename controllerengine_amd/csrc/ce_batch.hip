@@ -129,6 +129,9 @@ struct ce_batch {
   // default stream, which cannot be captured), ordered after / before the caller's stream by events
   hipStream_t own = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // mixed chains fork onto a second stream (see enqueue)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // tuning
   int autotune = 0;
   bool tuned = false;
@@ -475,23 +478,48 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   batch_mark(b, "fill", st);
   const ProblemDev* const* probs = (const ProblemDev* const*)b->ptrs.p;
   const StageBufs* bufs = (const StageBufs*)b->bufs.p;
+  // mixed: the nominal-loop eigensolve, float sweep and refinement run on the side stream, beside
+  // the simulation and the fit (per-launch timing keeps everything on one stream)
+  const bool fork = mixed && !b->profiling;
+  if (fork && !b->side) {
+    BCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    BCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    BCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  }
+  bool forked = false, joined = false;
   for (size_t i = 0; i < b->launches.size(); i++) {
     Launch& L = b->launches[i];
     const int total = L.start.back();
+    const bool on_side = fork && (L.kind == BK_EIG_NOM || L.kind == BK_SWEEP_MARK32 || L.kind == BK_REFINE64);
+    if (fork && !forked && L.kind != BK_EIG_LIN) {
+      BCHK(hipEventRecord(b->ev_fork, st));
+      BCHK(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+      forked = true;
+    }
+    if (fork && !joined && L.kind == BK_SCAN32) {
+      BCHK(hipEventRecord(b->ev_join, b->side));
+      BCHK(hipStreamWaitEvent(st, b->ev_join, 0));
+      joined = true;
+    }
     if (total == 0) continue;
+    hipStream_t ls = on_side ? b->side : st;
     int grid = grids ? (*grids)[i] : L.grid;
     grid = std::max(1, std::min(grid, total));
     if (L.kind == SIM_LAUNCH) {
-      if (!jit_launch_sim_batch(L.fn, grid, st, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
+      if (!jit_launch_sim_batch(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
                                 (const char* const*)b->ptrs.p, (int)L.pid.size(), total))
         throw std::runtime_error("launch of the batched simulation kernel failed");
     } else {
       BatchTab T{L.d_start, L.d_pid, probs, bufs, (int32_t)L.pid.size(), total};
-      const bool ok = (L.kind >= BK_ATB) ? launch_batch_mixed(L.kind, L.key, T, grid, st)
-                                          : launch_batch_fp64(L.kind, L.key, T, grid, st);
+      const bool ok = (L.kind >= BK_ATB) ? launch_batch_mixed(L.kind, L.key, T, grid, ls)
+                                          : launch_batch_fp64(L.kind, L.key, T, grid, ls);
       if (!ok) throw std::runtime_error("no batched kernel for " + L.label);
     }
     batch_mark(b, L.label, st);
+  }
+  if (fork && !joined) {
+    BCHK(hipEventRecord(b->ev_join, b->side));
+    BCHK(hipStreamWaitEvent(st, b->ev_join, 0));
   }
   launch_info_final(b->info64, d_out, b->total_info, st);
   if (d_out64) BCHK(hipMemcpyAsync(d_out64, b->info64, b->total_info * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -621,6 +649,9 @@ extern "C" CE_EXPORT void ce_batch_destroy(ce_batch* b) {
   if (b->ev_in) hipEventDestroy(b->ev_in);
   if (b->ev_out) hipEventDestroy(b->ev_out);
   if (b->own) hipStreamDestroy(b->own);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_join) hipEventDestroy(b->ev_join);
+  if (b->side) hipStreamDestroy(b->side);
   delete b;
 }
 
