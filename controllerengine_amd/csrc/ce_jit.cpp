@@ -52,8 +52,8 @@ struct Gen {
   std::string T() const { return f32 ? "float" : "double"; }
   std::string L(double v) const { return f32 ? litf((float)v) : lit(v); }
   std::string fma() const { return f32 ? "__builtin_fmaf" : "__builtin_fma"; }
-  std::string fmin() const { return f32 ? "__builtin_fminf" : "__builtin_fmin"; }
-  std::string fmax() const { return f32 ? "__builtin_fmaxf" : "__builtin_fmax"; }
+  std::string fmin() const { return f32 ? "ce_minf" : "ce_min"; }
+  std::string fmax() const { return f32 ? "ce_maxf" : "ce_max"; }
   std::string zero() const { return f32 ? "0.0f" : "0.0"; }
 
   // One instruction chain (reference src/main.cu:260-291) as straight-line code; returns the
@@ -87,7 +87,10 @@ struct Gen {
         case 3: o << "    " << res << " = " << a << " * " << b << ";\n"; break;
         case 4: o << "    " << res << " = " << a << " / " << b << ";\n"; break;
         case 5: o << "    " << res << " = " << fmin() << "(" << fmax() << "(" << prev << ", " << a << "), " << b << ");\n"; break;
-        case 6: o << "    " << res << " = (" << T() << ")((" << zero() << " < " << a << ") - (" << a << " < " << zero() << "));\n"; break;
+        case 6:  // (0 < a) - (a < 0): +-1 with a's sign, +0 for zero and NaN
+          o << "    " << res << " = (" << a << " < " << zero() << " || " << a << " > " << zero() << ") ? "
+            << (f32 ? "__builtin_copysignf(1.0f, " : "__builtin_copysign(1.0, ") << a << ") : " << zero() << ";\n";
+          break;
         default: break;
       }
       started = true;
@@ -141,7 +144,10 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   o << "  float* dy = A.dy;\n";
   o << "  const " << T << " amp = " << K(P.ref_amp, CE_OFF(ref_amp), CE_OFF(fref_amp)) << ";\n";
   if (structural) o << "  const char* Pq = Pp;\n";
-  o << "  for (int t = 0; t < TT; t++) {\n";
+  // one time step; the phase flags (fit window stored, steady-state sum, reference still zero,
+  // full-trajectory dump) are compile-time constants in the phase loops below, so a step carries
+  // no per-step branches besides the loop's own
+  o << "  auto step = [&](const int t, auto ST, auto SS, auto RZ, auto DY) __attribute__((always_inline)) {\n";
   if (structural)
     o << "    asm volatile(\"\" : \"+s\"(Pq));\n"
          "    Pq = (const char*)(((unsigned long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long)Pq >> 32)) << 32) |\n"
@@ -159,13 +165,13 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   };
   const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "", ins_cst(P.dc_ops, CE_OFF(dc_ops), CE_OFF(fdc_aval), CE_OFF(fdc_bval)));
   o << "    const " << T << " y = " << y << ";\n";
-  o << "    if (dy) dy[(long)c * TT + t] = (float)y;\n";
-  o << "    if (t >= f0 && t < f1) yb[(long)(t - f0) * n] = y;\n";
-  o << "    if (t >= s0) ysum = ysum + y;\n";
+  o << "    if (DY) dy[(long)c * TT + t] = (float)y;\n";
+  o << "    if (ST) yb[(long)(t - f0) * n] = y;\n";
+  o << "    if (SS) ysum = ysum + y;\n";
   o << "    ymax = " << g.fmax() << "(ymax, y); ymin = " << g.fmin() << "(ymin, y);\n";
   o << "    " << T << " kx = " << g.zero() << ";\n";
   for (int k = 0; k < ns; k++) o << "    kx = " << g.fma() << "(K" << k << ", xh" << k << ", kx);\n";
-  o << "    const " << T << " rr = (t < rs) ? " << g.zero() << " : amp;\n";
+  o << "    const " << T << " rr = RZ ? " << g.zero() << " : amp;\n";
   o << "    const " << T << " u_raw = " << g.fma() << "(N, rr, -kx);\n";
   const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", ins_cst(P.u_ops, CE_OFF(u_ops), CE_OFF(fu_aval), CE_OFF(fu_bval)));
   o << "    const " << T << " uu = " << u << ";\n";
@@ -237,8 +243,27 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
     }
   }
   for (int i = 0; i < nd; i++) o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
-  o << "  }\n";
+  o << "  };\n";
   in_loop = false;
+  // the horizon in segments between the phase boundaries; within one the flags are constant
+  o << "  const bool wdy = dy != nullptr;\n"
+       "  int t = 0;\n"
+       "  while (t < TT) {\n"
+       "    int e = TT;\n"
+       "    if (rs > t && rs < e) e = rs;\n"
+       "    if (f0 > t && f0 < e) e = f0;\n"
+       "    if (f1 > t && f1 < e) e = f1;\n"
+       "    if (s0 > t && s0 < e) e = s0;\n"
+       "    const bool st = t >= f0 && t < f1, ss = t >= s0, rz = t < rs;\n"
+       "    if (wdy) {\n"
+       "      for (; t < e; t++) step(t, st, ss, rz, true);\n"
+       "      continue;\n"
+       "    }\n"
+       "    switch ((st ? 4 : 0) | (ss ? 2 : 0) | (rz ? 1 : 0)) {\n";
+  for (int k = 0; k < 8; k++)
+    o << "      case " << k << ": for (; t < e; t++) step(t, BC<" << (k >> 2 & 1) << ">{}, BC<" << (k >> 1 & 1) << ">{}, BC<"
+      << (k & 1) << ">{}, BC<0>{}); break;\n";
+  o << "    }\n  }\n";
   o << "  if (unstable) return;\n";
   // yss and overshoot (reference src/modalsetup.cu:52-62)
   o << "  const " << T << " yss = ysum / (" << T << ")(TT - s0);\n";
@@ -254,8 +279,19 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
 
 static const char* kSimArgs =
     "struct SimArgs { const float* gains; const int* flags; const int* list; const int* count; void* ybuf;\n"
-    "  double* info; float* dy; int n; int pad; };\n";
+    "  double* info; float* dy; int n; int pad; };\n"
+    "template <int B> struct BC { __device__ constexpr operator bool() const { return B != 0; } };\n"
+    // fmin/fmax as the bare instructions: every operand here is an arithmetic result, a constant or
+    // an infinity, never a signalling NaN, so they return what fmin/fmax return, without the
+    // canonicalising moves the compiler adds for operands it cannot prove quiet
+    "static __device__ inline float ce_maxf(float a, float b) { float r; asm(\"v_max_f32 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
+    "static __device__ inline float ce_minf(float a, float b) { float r; asm(\"v_min_f32 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
+    "static __device__ inline double ce_max(double a, double b) { double r; asm(\"v_max_f64 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
+    "static __device__ inline double ce_min(double a, double b) { double r; asm(\"v_min_f64 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n";
 
+// Both forms raise the simulation waves' issue priority (s_setprio 3): the simulation is one serial
+// recurrence per candidate with a single wave on each SIMD it uses, and the kernels that run beside
+// it on a second stream take the issue slots it leaves.
 std::string gen_sim_source(const ProblemDev& P, bool mixed) {
   Gen g;
   g.f32 = mixed;
@@ -272,6 +308,7 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed) {
        "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
        "    float* __restrict__ dy, int n) {\n"
        "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n"
+       "  __builtin_amdgcn_s_setprio(3);\n"
        "  sim_one(A, nullptr, blockIdx.x);\n}\n";
   return o.str();
 }
@@ -291,6 +328,7 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
        "    const char* const* __restrict__ probs, int J, int total) {\n"
+       "  __builtin_amdgcn_s_setprio(3);\n"
        "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
        "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
        "    int lo = 0, hi = J - 1;\n"

@@ -183,13 +183,18 @@ __device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, con
   fr[3] = (double)yss;
 }
 
+// The simulation is one long serial recurrence per candidate and holds only one wave on the SIMDs
+// it uses; its waves take issue priority over the waves of kernels that run beside it on another
+// stream, which fill the remaining issue slots.
 template <typename R, int NS, int ND>
 __global__ __launch_bounds__(64) void k_sim(const ProblemDev* __restrict__ P, StageBufs B) {
+  __builtin_amdgcn_s_setprio(3);
   k_sim_body<R, NS, ND>(P, B, blockIdx.x, gridDim.x, 0, 0);
 }
 
 template <typename R, int NS, int ND>
 __global__ __launch_bounds__(64) void k_sim_batch(BatchTab T) {
+  __builtin_amdgcn_s_setprio(3);
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
     k_sim_body<R, NS, ND>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
