@@ -33,6 +33,7 @@ bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_gate_wait(const int* gate, int gen, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
 }  // namespace ce
@@ -81,7 +82,11 @@ struct ce_engine {
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
   // simulation and the fit (forked after the LIN_A eigensolve, joined before the crossing scan)
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // 1 when the last simulation ran the specialised kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // start signal of the gated simulation (one int, value = gate_gen of the last gated launch)
+  Buf gate;
+  int gate_gen = 0;
+  int n_simd = 0;  // SIMDs of the device (4 per CU)
 };
 
 static void* ensure(ce_engine::Buf& b, size_t bytes) {
@@ -232,35 +237,53 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     HIPCHK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
   }
-  if (fork) {
-    launch_eig_lin(H.nt, P, B, st);
-    HIPCHK(hipEventRecord(E->ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
-    launch_eig_nom(H.ns, P, B, E->side);
-    if (!launch_mark_refine(ns, F, P, B, E->side)) throw std::runtime_error("no sweep kernel for these dimensions");
-    HIPCHK(hipEventRecord(E->ev_join, E->side));
-  } else {
-    launch_eig_fp64(H.ns, H.nt, P, B, st);
+  // the specialised simulation kernel (compiled once per problem and form); with the second stream
+  // and a grid that fits the device's SIMDs at one wave each, the gated form
+  const int sim_grid = (n + 63) / 64;
+  if (fork && !E->n_simd) {
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device));
+    E->n_simd = 4 * cus;
   }
-  launch_compact(P, B, st);
-  HIPCHK(hipEventRecord(E->ev[1], st));
-  // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
-  void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;
+  const bool want_gate = fork && sim_grid <= E->n_simd;
   void* fn = nullptr;
-  E->last_sim_jit = 0;
+  bool gated = false;
   if (jit_enabled()) {
     if (!E->jit) E->jit = jit_cache_create();
     std::string jerr;
-    if (jit_get(E->jit, gen_sim_source(H, mixed), &fn, jerr)) {
-      if (!jit_launch_sim(fn, (n + 63) / 64, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n))
-        throw std::runtime_error("launch of the specialised simulation kernel failed");
-      kmark(mixed ? "ce_sim_f32" : "ce_sim_f64", st);
-      E->last_sim_jit = 1;
-    } else {
-      fn = nullptr;
-    }
+    if (want_gate && jit_get(E->jit, gen_sim_source(H, mixed, true), &fn, jerr)) gated = true;
+    else if (!jit_get(E->jit, gen_sim_source(H, mixed, false), &fn, jerr)) fn = nullptr;
   }
-  if (!fn) {
+  if (gated && !E->gate.p) {
+    ensure(E->gate, sizeof(int));
+    launch_fill_i32((int*)E->gate.p, 0, 1, st);
+  }
+  if (fork) launch_eig_lin(H.nt, P, B, st);
+  else launch_eig_fp64(H.ns, H.nt, P, B, st);
+  launch_compact(P, B, st);
+  if (fork) {
+    HIPCHK(hipEventRecord(E->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
+    if (gated) {
+      if (++E->gate_gen <= 0) E->gate_gen = 1;
+      launch_gate_wait((const int*)E->gate.p, E->gate_gen, E->side);
+    }
+    launch_eig_nom(H.ns, P, B, E->side);
+    if (!launch_mark_refine(ns, F, P, B, E->side)) throw std::runtime_error("no sweep kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev_join, E->side));
+  }
+  HIPCHK(hipEventRecord(E->ev[1], st));
+  // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
+  void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;
+  E->last_sim_jit = 0;
+  if (fn) {
+    const bool ok = gated ? jit_launch_sim_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info,
+                                                 B.d_y, n, (int*)E->gate.p, E->gate_gen)
+                          : jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n);
+    if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");
+    kmark(mixed ? "ce_sim_f32" : "ce_sim_f64", st);
+    E->last_sim_jit = 1;
+  } else {
     const bool ok = mixed ? launch_sim_mixed(ns, H.nd, P, B, st) : launch_sim_fp64(ns, H.nd, P, B, st);
     if (!ok) throw std::runtime_error("no simulation kernel for these dimensions");
   }
@@ -308,7 +331,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
                   &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
-                  &E->rcount, &E->atbp})
+                  &E->rcount, &E->atbp, &E->gate})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

@@ -292,11 +292,18 @@ static const char* kSimArgs =
 // Both forms raise the simulation waves' issue priority (s_setprio 3): the simulation is one serial
 // recurrence per candidate with a single wave on each SIMD it uses, and the kernels that run beside
 // it on a second stream take the issue slots it leaves.
-std::string gen_sim_source(const ProblemDev& P, bool mixed) {
+//
+// The gated form (a single launch with no more workgroups than the device has SIMDs) also
+//  * claims every register of its SIMD (512 = 256 VGPRs + 256 AGPRs), so no wave of another kernel
+//    shares a SIMD with a simulation wave and slows its serial chain;
+//  * stores gate_gen to *gate when its last workgroup starts: by then the workgroups before it have
+//    been placed, and a bounded waiter on the second stream lets the kernels there start, on the
+//    SIMDs the simulation leaves free.
+std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated) {
   Gen g;
   g.f32 = mixed;
   std::ostringstream& o = g.o;
-  const char* name = mixed ? "ce_sim_f32" : "ce_sim_f64";
+  const char* name = gated ? (mixed ? "ce_sim_f32_g" : "ce_sim_f64_g") : (mixed ? "ce_sim_f32" : "ce_sim_f64");
   o << "// kernel: " << name << "\n";
   o << "// generated: closed-loop simulation, ns=" << P.ns << " nd=" << P.nd << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
   o << kSimArgs;
@@ -306,9 +313,13 @@ std::string gen_sim_source(const ProblemDev& P, bool mixed) {
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
        "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
-       "    float* __restrict__ dy, int n) {\n"
-       "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n"
-       "  __builtin_amdgcn_s_setprio(3);\n"
+       "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "") << ") {\n"
+       "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
+  if (gated)
+    o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
+         "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
+         "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+  o << "  __builtin_amdgcn_s_setprio(3);\n"
        "  sim_one(A, nullptr, blockIdx.x);\n}\n";
   return o.str();
 }
@@ -415,6 +426,12 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n) {
   void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n};
+  return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
+}
+
+bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gate, &gen};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

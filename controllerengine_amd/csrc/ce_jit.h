@@ -14,11 +14,15 @@ JitCache* jit_cache_create();
 void jit_cache_destroy(JitCache* c);
 bool jit_enabled();
 // HIP source of the simulation kernel for problem P (mixed: float state and trajectory), every
-// constant written into the source.
-std::string gen_sim_source(const ProblemDev& P, bool mixed);
+// constant written into the source.  gated: the form that holds whole SIMDs and signals its start
+// (jit_launch_sim_gated; grid at most the device's SIMD count).
+std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated = false);
 bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err);
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n);
+// the gated form: its last workgroup stores gen to *gate when it starts
+bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen);
 
 // Batched form: one kernel per instruction structure and dimensions (sim_structure_key), constants
 // and sizes read from each problem's descriptor.  SimArgs mirrors the struct in the generated

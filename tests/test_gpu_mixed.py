@@ -76,3 +76,22 @@ def test_mixed_specialised_and_interpreted_agree(engine, monkeypatch):
     # for finite trajectories (a trajectory that overflows may end as inf instead of NaN)
     same = (a == b) | (np.isnan(a) & np.isnan(b))
     assert same.all(), "float simulation kernels differ in %d cells" % int((~same).sum())
+
+
+@pytest.mark.parametrize("name,n", [("c2", 4096), ("c3", 1024)])
+def test_mixed_two_streams_equal_one_stream(engine, name, n):
+    """The default mixed chain runs the nominal eigensolve, the sweep and the refinement on a second
+    stream beside a simulation that holds whole SIMDs and signals its start; with per-kernel timing on,
+    every kernel runs in order on one stream with the plain simulation kernel.  Same rows, bit for bit."""
+    p = workloads.CONFIGS[name](n=n)
+    print("case", name, n, flush=True)
+    _, two = engine.evaluate(p, PRECISION_MIXED)
+    engine.set_profiling(True)
+    try:
+        _, one = engine.evaluate(p, PRECISION_MIXED)
+    finally:
+        engine.set_profiling(False)
+    _, again = engine.evaluate(p, PRECISION_MIXED)
+    for other in (one, again):
+        same = (two == other) | (np.isnan(two) & np.isnan(other))
+        assert same.all(), "rows differ in %d cells" % int((~same).sum())
