@@ -265,7 +265,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     HIPCHK(hipEventRecord(E->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
     if (gated) {
-      if (++E->gate_gen <= 0) E->gate_gen = 1;
+      E->gate_gen = (E->gate_gen >= (1 << 30)) ? 1 : E->gate_gen + 1;
       launch_gate_wait((const int*)E->gate.p, E->gate_gen, E->side);
     }
     launch_eig_nom(H.ns, P, B, E->side);
