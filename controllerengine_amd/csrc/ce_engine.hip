@@ -227,9 +227,6 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   launch_info_init(d_info_in, B.info, nn * H.info_w, st);
   E->last_stream = st;
 
-  // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
-  MarkerScope marks(E, st);
-  HIPCHK(hipEventRecord(E->ev[0], st));
   // with per-kernel timing on, everything stays on one stream so that the events bracket kernels
   const bool fork = mixed && !E->profiling;
   if (fork && !E->side) {
@@ -254,6 +251,9 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     if (want_gate && jit_get(E->jit, gen_sim_source(H, mixed, true), &fn, jerr)) gated = true;
     else if (!jit_get(E->jit, gen_sim_source(H, mixed, false), &fn, jerr)) fn = nullptr;
   }
+  // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
+  MarkerScope marks(E, st);
+  HIPCHK(hipEventRecord(E->ev[0], st));
   if (gated && !E->gate.p) {
     ensure(E->gate, sizeof(int));
     launch_fill_i32((int*)E->gate.p, 0, 1, st);

@@ -5,7 +5,8 @@
 //   modal fit      k_fitcf: the normal equations of the damped-sinusoid fit (reference
 //                  src/modalsetup.cu:64-91) are formed in double without a pass over time for the
 //                  Gram: each entry sum_t e^{(s_e+s_f) tau} trig(w_e tau) trig(w_f tau) is a geometric
-//                  series, summed in closed form with expm1 (no cancellation for slow modes).  The
+//                  series, summed in closed form from per-mode expm1 values (no cancellation for
+//                  slow modes).  The
 //                  right-hand side sum_t phi(t)(y(t) - yss) runs over the stored trajectory with the
 //                  basis advanced by a double complex recurrence.  Then the same LDL^T solve and
 //                  score as the FP64 chain (src/modalsolve.cu).
@@ -48,21 +49,6 @@ __device__ __forceinline__ Cplx cexpm1(double ar, double ai) {
   cedm::sincos(0.5 * ai, sh, ch);
   const double em = cedm::expm1(ar);
   return Cplx{em * c - 2.0 * sh * sh, (em + 1.0) * s};
-}
-
-// sum_{t=0}^{N-1} exp((sig + i beta)(tau0 + t ts) - shift)
-__device__ __forceinline__ Cplx geo_sum(double sig, double beta, double tau0, double ts, int N, double shift) {
-  const double ar = sig * ts, ai = beta * ts;
-  Cplx ratio;
-  if (ar == 0.0 && ai == 0.0) {
-    ratio = Cplx{(double)N, 0.0};
-  } else {
-    ratio = cdiv(cexpm1((double)N * ar, (double)N * ai), cexpm1(ar, ai));
-  }
-  const double mag = cedm::exp(sig * tau0 - shift);
-  double s, c;
-  cedm::sincos(beta * tau0, s, c);
-  return cmul(Cplx{mag * c, mag * s}, ratio);
 }
 
 // Right-hand side of the fit, sum_t phi_e(t) (y(t) - yss), by Horner's rule in double:
@@ -174,27 +160,51 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
   const double ts = P->ts, t0 = P->fit_t0, t1 = P->fit_t1;
   const int f0 = P->fit_start_id, N = P->fit_count;
   const double tau0 = (double)f0 * ts;
-  if (active) {
-    if (g < u) {
-      // Atb_e = sum_parts phi_e(t_part) * partial_e,part (complex)
-      const int e = g;
-      const double re = B.mode_re[(size_t)e * n + c], im = B.mode_im[(size_t)e * n + c];
-      const double sh = cedm::fmax_(re * t0, re * t1);
-      double sr = 0.0, si = 0.0;
-      for (int q = 0; q < B.atb_parts; q++) {
-        const double tau = (double)(f0 + q * B.atb_chunk) * ts;
-        const double m0 = cedm::exp(re * tau - sh);
-        double s0, c0;
-        cedm::sincos(im * tau, s0, c0);
-        const double br = m0 * c0, bi = m0 * s0;
-        const double2 pv = B.atbp[((size_t)q * nt + e) * n + idx];
-        sr = sr + (br * pv.x - bi * pv.y);
-        si = si + (br * pv.y + bi * pv.x);
-      }
-      vec[slot][0][2 * e] = sr;
-      vec[slot][0][2 * e + 1] = si;
+  // per-mode factors of the closed form (modes of this candidate in vec[1..4], which the solve
+  // below uses as scratch only after the Gram is complete): PM(q, e), q = E1 re/im, EN re/im,
+  // W re/im, x1, xN
+  constexpr int NM = NCP / 2;
+  double* pm = &vec[slot][1][0];
+  auto PM = [&](int q, int e) -> double& { return pm[q * NM + e]; };
+  if (active && g < u) {
+    // Atb_e = sum_parts phi_e(t_part) * partial_e,part (complex)
+    const int e = g;
+    const double re = B.mode_re[(size_t)e * n + c], im = B.mode_im[(size_t)e * n + c];
+    const double sh = cedm::fmax_(re * t0, re * t1);
+    double sr = 0.0, si = 0.0;
+    for (int q = 0; q < B.atb_parts; q++) {
+      const double tau = (double)(f0 + q * B.atb_chunk) * ts;
+      const double m0 = cedm::exp(re * tau - sh);
+      double s0, c0;
+      cedm::sincos(im * tau, s0, c0);
+      const double br = m0 * c0, bi = m0 * s0;
+      const double2 pv = B.atbp[((size_t)q * nt + e) * n + idx];
+      sr = sr + (br * pv.x - bi * pv.y);
+      si = si + (br * pv.y + bi * pv.x);
     }
-    // Gram in closed form: mode pairs (e >= f) spread over the group's lanes
+    vec[slot][0][2 * e] = sr;
+    vec[slot][0][2 * e + 1] = si;
+    const double ar = re * ts, ai = im * ts;
+    const Cplx e1 = cexpm1(ar, ai), en = cexpm1((double)N * ar, (double)N * ai);
+    const double mag = cedm::exp(re * tau0 - sh);
+    double sw, cw;
+    cedm::sincos(im * tau0, sw, cw);
+    PM(0, e) = e1.r;
+    PM(1, e) = e1.i;
+    PM(2, e) = en.r;
+    PM(3, e) = en.i;
+    PM(4, e) = mag * cw;
+    PM(5, e) = mag * sw;
+    PM(6, e) = cedm::expm1(2.0 * ar);
+    PM(7, e) = cedm::expm1(2.0 * ((double)N * ar));
+  }
+  __syncthreads();
+  if (active) {
+    // Gram in closed form: mode pairs (e >= f) spread over the group's lanes.  Each entry pairs
+    // sums of e^{(z_e + z_f) t} and e^{(z_e + conj z_f) t}, geometric series whose ratio
+    // expm1(N z) / expm1(z) is formed from the per-mode expm1 values:
+    // expm1(a + b) = A + B + A B with A = expm1(a), B = expm1(b) (no cancellation: the real parts
+    // of stable modes share their sign), and for e = f the difference exponent is the real 2 s_e ts.
     const int npairs = u * (u + 1) / 2;
     for (int p = g; p < npairs; p += G) {
       int ea = 0;
@@ -202,9 +212,21 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
       const int fb = p - ea * (ea + 1) / 2;
       const double se = B.mode_re[(size_t)ea * n + c], we = B.mode_im[(size_t)ea * n + c];
       const double sf = B.mode_re[(size_t)fb * n + c], wf = B.mode_im[(size_t)fb * n + c];
-      const double shift = cedm::fmax_(se * t0, se * t1) + cedm::fmax_(sf * t0, sf * t1);
-      const Cplx sdif = geo_sum(se + sf, we - wf, tau0, ts, N, shift);
-      const Cplx ssum = geo_sum(se + sf, we + wf, tau0, ts, N, shift);
+      const Cplx E1e{PM(0, ea), PM(1, ea)}, ENe{PM(2, ea), PM(3, ea)}, We{PM(4, ea), PM(5, ea)};
+      const Cplx E1f{PM(0, fb), PM(1, fb)}, ENf{PM(2, fb), PM(3, fb)}, Wf{PM(4, fb), PM(5, fb)};
+      const Cplx E1c{E1f.r, -E1f.i}, ENc{ENf.r, -ENf.i}, Wc{Wf.r, -Wf.i};
+      auto xm1 = [](Cplx a, Cplx b) {  // expm1(x + y) from A = expm1(x), B = expm1(y)
+        const Cplx ab = cmul(a, b);
+        return Cplx{(a.r + b.r) + ab.r, (a.i + b.i) + ab.i};
+      };
+      Cplx rd, rs;
+      if (se + sf == 0.0 && we - wf == 0.0) rd = Cplx{(double)N, 0.0};
+      else if (ea == fb) rd = Cplx{PM(7, ea) / PM(6, ea), 0.0};
+      else rd = cdiv(xm1(ENe, ENc), xm1(E1e, E1c));
+      if (se + sf == 0.0 && we + wf == 0.0) rs = Cplx{(double)N, 0.0};
+      else rs = cdiv(xm1(ENe, ENf), xm1(E1e, E1f));
+      const Cplx sdif = cmul(cmul(We, Wc), rd);
+      const Cplx ssum = cmul(cmul(We, Wf), rs);
       const double cc = 0.5 * (sdif.r + ssum.r), ss = 0.5 * (sdif.r - ssum.r);
       const double cs = 0.5 * (ssum.i - sdif.i), sc = 0.5 * (ssum.i + sdif.i);
       gram[slot][tri(2 * ea, 2 * fb)] = cc;
