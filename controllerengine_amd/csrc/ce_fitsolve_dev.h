@@ -20,14 +20,18 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 // workgroup-scope fence (single-wave workgroups).
 // G2: packed lower triangle of the 2nt x 2nt Gram (overwritten), atb: right-hand side; dg, sd, cs,
 // x, tmp: LDS vectors of 2nt; trans: LDS ints of 2nt.  u: unique modes of candidate c.
+// n2: order of the system solved, 2nt, or 2u when the rows and columns of the modes u..nt-1 are
+// exactly zero (then they never take part: no pivot is chosen among them before the last nonzero
+// one, their eliminations and substitutions leave zeros, and the leading 2u x 2u solve returns the
+// same values).
 __device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 template <int G>
 __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
                                                     double* row, double* G2, const double* atb, double* dg,
                                                     double* sd, double* cs, double* x, double* tmp, int* trans,
-                                                    int l) {
-  const int nt = P->nt, N2 = 2 * nt;
+                                                    int l, int n2) {
+  const int nt = P->nt, N2 = n2;
   const int n = B.n;
   double* sc = row + P->info_scoring;
   for (int i = l; i < N2; i += G) {

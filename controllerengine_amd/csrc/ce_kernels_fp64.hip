@@ -408,7 +408,7 @@ __device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P
   }
   double* sc = row + P->info_scoring;
   if (sc[0] != 0.0) return;  // modalsolve skips flagged candidates (src/modalsolve.cu:112)
-  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g);
+  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g, 2 * nt);
 }
 
 template <int NB>

@@ -250,7 +250,7 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
   }
   if (row[P->info_scoring] != 0.0) return;
   solve_and_score_grp<G>(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4],
-                         vec[slot][5], itr[slot], g);
+                         vec[slot][5], itr[slot], g, 2 * u);
 }
 
 template <int NB>
