@@ -14,9 +14,11 @@
 //
 // The result is therefore bit-identical to the sequential solver.  Groups of one wave may diverge
 // from each other; a group never waits on another.  Phases that read what other lanes of the group
-// wrote are separated by grp_sync(): a workgroup-scope fence, which keeps the compiler from moving
-// LDS accesses across it and waits for this wave's LDS operations.  Kernels using it run one wave
-// per workgroup, and a wave's LDS operations complete in issue order.
+// wrote are separated by grp_sync(): a wavefront-scope fence, which keeps the compiler from moving
+// LDS accesses across it.  The lanes of a group are lanes of one wave, and a wave's memory
+// operations are seen by all of its lanes in issue order, so the fence needs no wait instruction
+// (the AMDGPU memory model emits none at wavefront scope).  Kernels using it run one wave per
+// workgroup.
 #pragma once
 
 #include <float.h>
@@ -25,7 +27,7 @@
 
 namespace ce {
 
-__device__ __forceinline__ void grp_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+__device__ __forceinline__ void grp_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 struct GHh {
   double e0, e1, tau, beta;

@@ -17,14 +17,15 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 // same operations in the same order as in a sequential solve (the pivot search and the
 // triangular substitutions run on every lane / on lane 0 in the original order), so the result
 // does not depend on G.  The lanes of the group must be convergent; phases are separated by a
-// workgroup-scope fence (single-wave workgroups).
+// wavefront-scope fence (the group's lanes are lanes of one wave: no wait is needed, only the
+// compiler's program order).
 // G2: packed lower triangle of the 2nt x 2nt Gram (overwritten), atb: right-hand side; dg, sd, cs,
 // x, tmp: LDS vectors of 2nt; trans: LDS ints of 2nt.  u: unique modes of candidate c.
 // n2: order of the system solved, 2nt, or 2u when the rows and columns of the modes u..nt-1 are
 // exactly zero (then they never take part: no pivot is chosen among them before the last nonzero
 // one, their eliminations and substitutions leave zeros, and the leading 2u x 2u solve returns the
 // same values).
-__device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+__device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 template <int G>
 __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
