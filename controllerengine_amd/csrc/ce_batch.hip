@@ -200,7 +200,8 @@ static void build_blobs(ce_batch* b, int precision) {
   }
 }
 
-static int g_eig(int m) { return m <= 4 ? 4 : (m <= 8 ? 8 : 16); }
+// eigen kernels: key = matrix capacity (4, 8, 16), candidates per workgroup = 64 / lanes
+static int eig_cpw(int m) { return 64 / eig_lanes(eig_key(m)); }
 
 // Compiled simulation kernels are shared by every batch of the process on the same device (the
 // caches are thread-safe); modules stay loaded until the process ends.
@@ -351,8 +352,8 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   };
   auto cdiv = [](int a, int c) { return (a + c - 1) / c; };
   using PP = ce_batch::Prob;
-  group(BK_EIG_LIN, [&](const PP& p) { return g_eig(p.nt); }, [&](const PP& p, const PS&) { return cdiv(p.n, 64 / g_eig(p.nt)); }, "k_eig_lin");
-  group(BK_EIG_NOM, [&](const PP& p) { return g_eig(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, 64 / g_eig(2 * p.ns)); }, "k_eig_nom");
+  group(BK_EIG_LIN, [&](const PP& p) { return eig_key(p.nt); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(p.nt)); }, "k_eig_lin");
+  group(BK_EIG_NOM, [&](const PP& p) { return eig_key(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(2 * p.ns)); }, "k_eig_nom");
   group(BK_COMPACT, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_compact");
   {  // simulation: one group per structure
     std::map<std::string, Launch> g;

@@ -54,12 +54,14 @@ __device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int 
   return h;
 }
 
-// Eigenvalues of the n x n matrix T (LDS, row stride S, destroyed) into WR/WI (LDS, n entries).
-// V: LDS scratch of at least n doubles.  l: lane index in the group.  Returns (in every lane)
-// 0 = success, 1 = no convergence within 40 n iterations, 2 = non-finite eigenvalue.
-template <int G>
+// Eigenvalues of the n x n matrix T (LDS, row stride S = G R + 1, destroyed) into WR/WI (LDS, n
+// entries), n <= G R.  Lane l of the group owns the rows / columns l, l + G, ..., l + (R-1) G.
+// V: LDS scratch of at least n doubles.  Returns (in every lane) 0 = success, 1 = no convergence
+// within 40 n iterations, 2 = non-finite eigenvalue.
+#define CE_LANES(j) _Pragma("unroll") for (int q_ = 0, j = l; q_ < R; q_++, j += G)
+template <int G, int R = 1>
 __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l) {
-  constexpr int S = G + 1;
+  constexpr int S = G * R + 1;
   double scale = 0.0;
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) {
@@ -68,12 +70,12 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
     }
   grp_sync();
   if (scale < DBL_MIN) {
-    if (l < n) { WR[l] = 0.0; WI[l] = 0.0; }
+    CE_LANES(j) if (j < n) { WR[j] = 0.0; WI[j] = 0.0; }
     grp_sync();
     return 0;
   }
-  if (l < n)
-    for (int i = 0; i < n; i++) T[i * S + l] = T[i * S + l] / scale;
+  CE_LANES(j) if (j < n)
+    for (int i = 0; i < n; i++) T[i * S + j] = T[i * S + j] / scale;
   grp_sync();
 
   // Hessenberg reduction, column by column
@@ -94,21 +96,20 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
       tau = (beta - c0) / beta;
     }
     grp_sync();
-    if (l >= 1 && l < rem) {
-      V[l - 1] = flat ? 0.0 : T[(i + 1 + l) * S + i] / den;
-      T[(i + 1 + l) * S + i] = 0.0;
+    CE_LANES(k) if (k >= 1 && k < rem) {
+      V[k - 1] = flat ? 0.0 : T[(i + 1 + k) * S + i] / den;
+      T[(i + 1 + k) * S + i] = 0.0;
     }
     if (l == 0) T[(i + 1) * S + i] = beta;
     grp_sync();
     if (rem == 1) {
-      if (l < n) {
-        if (l == i + 1) T[(i + 1) * S + i + 1] = T[(i + 1) * S + i + 1] * (1.0 - tau);
-        T[l * S + i + 1] = T[l * S + i + 1] * (1.0 - tau);
+      CE_LANES(j) if (j < n) {
+        if (j == i + 1) T[(i + 1) * S + i + 1] = T[(i + 1) * S + i + 1] * (1.0 - tau);
+        T[j * S + i + 1] = T[j * S + i + 1] * (1.0 - tau);
       }
       grp_sync();
     } else if (tau != 0.0) {
-      if (l > i && l < n) {  // left: rows i+1..n-1 of column l
-        const int j = l;
+      CE_LANES(j) if (j > i && j < n) {  // left: rows i+1..n-1 of column j
         double tmp = 0.0;
         for (int k = 0; k < rem - 1; k++) tmp = cedm::fma_(V[k], T[(i + 2 + k) * S + j], tmp);
         tmp = tmp + T[(i + 1) * S + j];
@@ -119,8 +120,8 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
         }
       }
       grp_sync();
-      if (l < n) {  // right: columns i+1..n-1 of row l
-        double* row = T + l * S + i + 1;
+      CE_LANES(r) if (r < n) {  // right: columns i+1..n-1 of row r
+        double* row = T + r * S + i + 1;
         double tmp = 0.0;
         for (int k = 0; k < rem - 1; k++) tmp = cedm::fma_(row[1 + k], V[k], tmp);
         tmp = tmp + row[0];
@@ -195,16 +196,16 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             s = -1.0 / u;
             c = -t * s;
           }
-          if (l >= iu - 1 && l < n) {
-            const double x = T[(iu - 1) * S + l], y = T[iu * S + l];
-            T[(iu - 1) * S + l] = c * x - s * y;
-            T[iu * S + l] = s * x + c * y;
+          CE_LANES(j) if (j >= iu - 1 && j < n) {
+            const double x = T[(iu - 1) * S + j], y = T[iu * S + j];
+            T[(iu - 1) * S + j] = c * x - s * y;
+            T[iu * S + j] = s * x + c * y;
           }
           grp_sync();
-          if (l <= iu) {
-            const double x = T[l * S + iu - 1], y = T[l * S + iu];
-            T[l * S + iu - 1] = c * x - s * y;
-            T[l * S + iu] = s * x + c * y;
+          CE_LANES(r) if (r <= iu) {
+            const double x = T[r * S + iu - 1], y = T[r * S + iu];
+            T[r * S + iu - 1] = c * x - s * y;
+            T[r * S + iu] = s * x + c * y;
           }
           grp_sync();
           if (l == 0) T[iu * S + iu - 1] = 0.0;
@@ -221,7 +222,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           exshift = exshift + sh0;
           const double s = cedm::fabs_(T[iu * S + iu - 1]) + cedm::fabs_(T[(iu - 1) * S + iu - 2]);
           grp_sync();
-          if (l <= iu) T[l * S + l] = T[l * S + l] - sh0;
+          CE_LANES(j) if (j <= iu) T[j * S + j] = T[j * S + j] - sh0;
           grp_sync();
           sh0 = 0.75 * s;
           sh1 = 0.75 * s;
@@ -237,7 +238,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             s = sh0 - sh2 / s;
             exshift = exshift + s;
             grp_sync();
-            if (l <= iu) T[l * S + l] = T[l * S + l] - s;
+            CE_LANES(j) if (j <= iu) T[j * S + j] = T[j * S + j] - s;
             grp_sync();
             sh0 = sh1 = sh2 = 0.964;
           }
@@ -273,8 +274,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
               else if (!first) T[k * S + k - 1] = h.beta;
             }
             if (h.tau != 0.0) {
-              if (l >= k && l < n) {  // left: rows k..k+2 of column l
-                const int j = l;
+              CE_LANES(j) if (j >= k && j < n) {  // left: rows k..k+2 of column j
                 double tmp = cedm::fma_(h.e0, T[(k + 1) * S + j], 0.0);
                 tmp = cedm::fma_(h.e1, T[(k + 2) * S + j], tmp);
                 tmp = tmp + T[k * S + j];
@@ -284,8 +284,8 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
               }
               grp_sync();
               const int r1 = ((iu < k + 3) ? iu : k + 3) + 1;
-              if (l < r1) {  // right: columns k..k+2 of row l
-                double* row = T + l * S + k;
+              CE_LANES(r) if (r < r1) {  // right: columns k..k+2 of row r
+                double* row = T + r * S + k;
                 double tmp = cedm::fma_(row[1], h.e0, 0.0);
                 tmp = cedm::fma_(row[2], h.e1, tmp);
                 tmp = tmp + row[0];
@@ -304,16 +304,15 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             grp_sync();
             if (l == 0) T[(iu - 1) * S + iu - 2] = h.beta;
             if (h.tau != 0.0) {
-              if (l >= iu - 1 && l < n) {
-                const int j = l;
+              CE_LANES(j) if (j >= iu - 1 && j < n) {
                 double tmp = cedm::fma_(h.e0, T[iu * S + j], 0.0);
                 tmp = tmp + T[(iu - 1) * S + j];
                 T[(iu - 1) * S + j] = T[(iu - 1) * S + j] - h.tau * tmp;
                 T[iu * S + j] = T[iu * S + j] - (h.tau * h.e0) * tmp;
               }
               grp_sync();
-              if (l < iu + 1) {
-                double* row = T + l * S + iu - 1;
+              CE_LANES(r) if (r < iu + 1) {
+                double* row = T + r * S + iu - 1;
                 double tmp = cedm::fma_(row[1], h.e0, 0.0);
                 tmp = tmp + row[0];
                 row[0] = row[0] - h.tau * tmp;
@@ -324,17 +323,17 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             grp_sync();
           }
         }
-        if (l >= im + 2 && l <= iu) {
-          T[l * S + l - 2] = 0.0;
-          if (l > im + 2) T[l * S + l - 3] = 0.0;
+        CE_LANES(r) if (r >= im + 2 && r <= iu) {
+          T[r * S + r - 2] = 0.0;
+          if (r > im + 2) T[r * S + r - 3] = 0.0;
         }
         grp_sync();
       }
     }
   }
   grp_sync();
-  if (l < n)
-    for (int i = 0; i < n; i++) T[i * S + l] = T[i * S + l] * scale;
+  CE_LANES(j) if (j < n)
+    for (int i = 0; i < n; i++) T[i * S + j] = T[i * S + j] * scale;
   grp_sync();
   int rc = 0;
   int i = 0;
@@ -363,5 +362,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
   grp_sync();
   return rc;
 }
+
+#undef CE_LANES
 
 }  // namespace ce

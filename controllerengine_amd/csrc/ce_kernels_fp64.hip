@@ -68,80 +68,95 @@ __global__ __launch_bounds__(64) void k_compact_batch(BatchTab T) {
 // modes with the adjacent-duplicate rule (src/modalsetup.cu:8-47), the unstable flag
 // (src/modalsetup.cu:26) and the nominal closed loop of the margins stage (src/allmargin.cu:61-79).
 //
-// G lanes per candidate, 64 / G candidates per single-wave workgroup; lane l builds row l of the
-// matrix and owns column / row l in the eigensolver.  k_eig_lin (G >= nt) solves LIN_A and derives
-// modes and flags; k_eig_nom (G >= 2 ns) solves the nominal loop of the candidates that need it.
-template <int G>
+// G lanes per candidate, 64 / G candidates per single-wave workgroup, R rows per lane (order up to
+// G R): lane l builds rows l, l + G, ... of the matrix and owns those columns / rows in the
+// eigensolver.  k_eig_lin (G R >= nt) solves LIN_A and derives modes and flags; k_eig_nom
+// (G R >= 2 ns) solves the nominal loop of the candidates that need it.
+#define CE_ROWS(r) _Pragma("unroll") for (int q_ = 0, r = l; q_ < R; q_++, r += G)
+template <int G, int R>
 __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
-  constexpr int S = G + 1, CPW = 64 / G;
-  __shared__ double smat[CPW][G * S];
-  __shared__ double svec[CPW][5 * G];
+  constexpr int N = G * R, S = N + 1, CPW = 64 / G;
+  __shared__ double smat[CPW][N * S];
+  __shared__ double svec[CPW][5 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
   const int c = bx * CPW + gi;
   if (c >= B.n) return;
   double* T = smat[gi];
   double* V = svec[gi];
-  double* WR = V + G;
-  double* WI = V + 2 * G;
-  double* MAG = V + 3 * G;
-  double* IM = V + 4 * G;
+  double* WR = V + N;
+  double* WI = V + 2 * N;
+  double* MAG = V + 3 * N;
+  double* IM = V + 4 * N;
   const int ns = P->ns, nd = P->nd, nt = P->nt, n = B.n;
   const float* g = B.gains + (size_t)c * P->cuda_w;
-  if (l < ns) {
-    const double Ll = g[ns + l];
-    double* oa = B.open_a + (size_t)c * ns * ns + (size_t)l * ns;
-    for (int j = 0; j < ns; j++) {
-      const double Kj = g[j];
-      const double bk = P->sb[l] * Kj;
-      const double cdk = P->sc[j] - P->sd * Kj;
-      const double lcd = Ll * cdk;
-      const double v = (P->sa[l * ns + j] - bk) - lcd;
-      T[l * S + j] = v;
-      oa[j] = v;
+  CE_ROWS(r) {
+    if (r < ns) {
+      const double Ll = g[ns + r];
+      double* oa = B.open_a + (size_t)c * ns * ns + (size_t)r * ns;
+      for (int j = 0; j < ns; j++) {
+        const double Kj = g[j];
+        const double bk = P->sb[r] * Kj;
+        const double cdk = P->sc[j] - P->sd * Kj;
+        const double lcd = Ll * cdk;
+        const double v = (P->sa[r * ns + j] - bk) - lcd;
+        T[r * S + j] = v;
+        oa[j] = v;
+      }
+      for (int j = 0; j < nd; j++) T[r * S + ns + j] = Ll * P->dc[j];
+    } else if (r < nt) {
+      const int i = r - ns;
+      for (int j = 0; j < ns; j++) T[r * S + j] = -(P->dbu[i] * (double)g[j]);
+      for (int j = 0; j < nd; j++) T[r * S + ns + j] = P->da[i * nd + j];
     }
-    for (int j = 0; j < nd; j++) T[l * S + ns + j] = Ll * P->dc[j];
-  } else if (l < nt) {
-    const int i = l - ns;
-    for (int j = 0; j < ns; j++) T[l * S + j] = -(P->dbu[i] * (double)g[j]);
-    for (int j = 0; j < nd; j++) T[l * S + ns + j] = P->da[i * nd + j];
   }
-  if (B.diag && B.d_lina && l < nt) {
-    float* dl = B.d_lina + (size_t)c * nt * nt + (size_t)l * nt;
-    for (int j = 0; j < nt; j++) dl[j] = (float)T[l * S + j];
-  }
-  grp_sync();
-  const int rc = eig_grp<G>(T, V, WR, WI, nt, l);
-  if (rc != 0 && l < nt) { WR[l] = cedm::qnan(); WI[l] = cedm::qnan(); }
-  grp_sync();
-  double re = 0.0, im = 0.0;
-  if (l < nt) {
-    const double wr = WR[l], wi = WI[l];
-    if (B.diag && B.d_eig) {
-      float* de = B.d_eig + (size_t)c * 2 * nt;
-      de[2 * l] = (float)wr;
-      de[2 * l + 1] = (float)wi;
+  if (B.diag && B.d_lina) {
+    CE_ROWS(r) if (r < nt) {
+      float* dl = B.d_lina + (size_t)c * nt * nt + (size_t)r * nt;
+      for (int j = 0; j < nt; j++) dl[j] = (float)T[r * S + j];
     }
-    const double mag = cedm::hypot(wr, wi);
-    MAG[l] = mag;
-    re = cedm::log(cedm::fmax_(mag, EPS30)) / P->ts;
-    im = cedm::atan2(wi, wr) / P->ts;
-    V[l] = re;
-    IM[l] = im;
   }
   grp_sync();
-  bool keep = false;
-  if (l < nt) {
-    const double pre = (l == 0) ? BIG10 : V[l - 1];
-    const double pim = (l == 0) ? BIG10 : IM[l - 1];
-    const bool dup = cedm::fabs_(re - pre) < EPS6 && cedm::fabs_(cedm::fabs_(im) - cedm::fabs_(pim)) < EPS6;
-    keep = !dup;
+  const int rc = eig_grp<G, R>(T, V, WR, WI, nt, l);
+  if (rc != 0) CE_ROWS(r) if (r < nt) { WR[r] = cedm::qnan(); WI[r] = cedm::qnan(); }
+  grp_sync();
+  double re[R], im[R];
+  CE_ROWS(r) {
+    re[q_] = 0.0;
+    im[q_] = 0.0;
+    if (r < nt) {
+      const double wr = WR[r], wi = WI[r];
+      if (B.diag && B.d_eig) {
+        float* de = B.d_eig + (size_t)c * 2 * nt;
+        de[2 * r] = (float)wr;
+        de[2 * r + 1] = (float)wi;
+      }
+      const double mag = cedm::hypot(wr, wi);
+      MAG[r] = mag;
+      re[q_] = cedm::log(cedm::fmax_(mag, EPS30)) / P->ts;
+      im[q_] = cedm::atan2(wi, wr) / P->ts;
+      V[r] = re[q_];
+      IM[r] = im[q_];
+    }
   }
+  grp_sync();
+  // modes in eigenvalue order: round q_ covers indices [q_ G, q_ G + G) in lane order
   const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
-  const unsigned long long kb = __ballot(keep) & gmask;
-  if (keep) {
-    const int u = __popcll(kb & ((1ull << threadIdx.x) - 1ull));
-    B.mode_re[(size_t)u * n + c] = re;
-    B.mode_im[(size_t)u * n + c] = cedm::fabs_(im);
+  int ubase = 0;
+  CE_ROWS(r) {
+    bool keep = false;
+    if (r < nt) {
+      const double pre = (r == 0) ? BIG10 : V[r - 1];
+      const double pim = (r == 0) ? BIG10 : IM[r - 1];
+      const bool dup = cedm::fabs_(re[q_] - pre) < EPS6 && cedm::fabs_(cedm::fabs_(im[q_]) - cedm::fabs_(pim)) < EPS6;
+      keep = !dup;
+    }
+    const unsigned long long kb = __ballot(keep) & gmask;
+    if (keep) {
+      const int u = ubase + __popcll(kb & ((1ull << threadIdx.x) - 1ull));
+      B.mode_re[(size_t)u * n + c] = re[q_];
+      B.mode_im[(size_t)u * n + c] = cedm::fabs_(im[q_]);
+    }
+    ubase += __popcll(kb);
   }
   if (l == 0) {
     double max_mag = 0.0;
@@ -154,31 +169,31 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
       flags |= F_UNSTABLE;
     }
     if ((flags & F_UNSTABLE) || sc[0] >= P->unstable_err) flags |= F_NOMINAL_SKIPPED;
-    B.uniq[c] = __popcll(kb);
+    B.uniq[c] = ubase;
     B.flags[c] = flags;
   }
 }
 
-template <int G>
+template <int G, int R>
 __global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_eig_lin_body<G>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+  k_eig_lin_body<G, R>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int G>
-__global__ __launch_bounds__(64) void k_eig_lin_batch(BatchTab T) {
+template <int G, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_lin_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
-    k_eig_lin_body<G>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+    k_eig_lin_body<G, R>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
 }
 
 
 // nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79)
-template <int G>
+template <int G, int R>
 __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
-  constexpr int S = G + 1, CPW = 64 / G;
-  __shared__ double smat[CPW][G * S];
-  __shared__ double svec[CPW][3 * G];
+  constexpr int N = G * R, S = N + 1, CPW = 64 / G;
+  __shared__ double smat[CPW][N * S];
+  __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
   const int c = bx * CPW + gi;
   if (c >= B.n) return;
@@ -186,40 +201,44 @@ __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P,
   if (flags & F_NOMINAL_SKIPPED) return;
   double* T = smat[gi];
   double* V = svec[gi];
-  double* WR = V + G;
-  double* WI = V + 2 * G;
+  double* WR = V + N;
+  double* WI = V + 2 * N;
   const int ns = P->ns, m = 2 * ns;
   const float* g = B.gains + (size_t)c * P->cuda_w;
-  if (l < ns) {
-    const double Ll = g[ns + l];
-    const double* oa = B.open_a + (size_t)c * ns * ns + (size_t)l * ns;
-    for (int j = 0; j < ns; j++) T[l * S + j] = oa[j] - P->pd * (Ll * (double)g[j]);
-    for (int j = 0; j < ns; j++) T[l * S + ns + j] = -(Ll * P->pc[j]);
-  } else if (l < m) {
-    const int i = l - ns;
-    for (int j = 0; j < ns; j++) T[l * S + j] = P->pb[i] * (double)g[j];
-    for (int j = 0; j < ns; j++) T[l * S + ns + j] = P->pa[i * ns + j];
+  CE_ROWS(r) {
+    if (r < ns) {
+      const double Ll = g[ns + r];
+      const double* oa = B.open_a + (size_t)c * ns * ns + (size_t)r * ns;
+      for (int j = 0; j < ns; j++) T[r * S + j] = oa[j] - P->pd * (Ll * (double)g[j]);
+      for (int j = 0; j < ns; j++) T[r * S + ns + j] = -(Ll * P->pc[j]);
+    } else if (r < m) {
+      const int i = r - ns;
+      for (int j = 0; j < ns; j++) T[r * S + j] = P->pb[i] * (double)g[j];
+      for (int j = 0; j < ns; j++) T[r * S + ns + j] = P->pa[i * ns + j];
+    }
   }
   grp_sync();
-  const int rc = eig_grp<G>(T, V, WR, WI, m, l);
-  const bool out = rc == 0 && l < m && cedm::hypot(WR[l], WI[l]) >= 1.0;
+  const int rc = eig_grp<G, R>(T, V, WR, WI, m, l);
+  bool out = false;
+  CE_ROWS(r) out = out || (rc == 0 && r < m && cedm::hypot(WR[r], WI[r]) >= 1.0);
   const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
   const bool bad = rc != 0 || (__ballot(out) & gmask) != 0ull;
   if (l == 0 && bad) B.flags[c] = flags | F_NOMINAL_UNSTABLE;
 }
 
-template <int G>
+template <int G, int R>
 __global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_eig_nom_body<G>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+  k_eig_nom_body<G, R>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int G>
-__global__ __launch_bounds__(64) void k_eig_nom_batch(BatchTab T) {
+template <int G, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_nom_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
-    k_eig_nom_body<G>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+    k_eig_nom_body<G, R>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
 }
+#undef CE_ROWS
 
 
 // ------------------------------------------------------------------------------------------------
@@ -560,17 +579,21 @@ bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hi
 }
 
 void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  if (nt <= 4) hipLaunchKernelGGL(k_eig_lin<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
-  else if (nt <= 8) hipLaunchKernelGGL(k_eig_lin<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
-  else hipLaunchKernelGGL(k_eig_lin<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  const int key = eig_key(nt), cpw = 64 / eig_lanes(key);
+  const dim3 grid((B.n + cpw - 1) / cpw);
+  if (key == 4) hipLaunchKernelGGL((k_eig_lin<4, 1>), grid, dim3(64), 0, st, P, B);
+  else if (key == 8) hipLaunchKernelGGL((k_eig_lin<8, 1>), grid, dim3(64), 0, st, P, B);
+  else hipLaunchKernelGGL((k_eig_lin<EIG16_G, 16 / EIG16_G>), grid, dim3(64), 0, st, P, B);
   kmark("k_eig_lin", st);
 }
 
 void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   const int m = 2 * ns;
-  if (m <= 4) hipLaunchKernelGGL(k_eig_nom<4>, dim3((B.n + 15) / 16), dim3(64), 0, st, P, B);
-  else if (m <= 8) hipLaunchKernelGGL(k_eig_nom<8>, dim3((B.n + 7) / 8), dim3(64), 0, st, P, B);
-  else hipLaunchKernelGGL(k_eig_nom<16>, dim3((B.n + 3) / 4), dim3(64), 0, st, P, B);
+  const int key = eig_key(m), cpw = 64 / eig_lanes(key);
+  const dim3 grid((B.n + cpw - 1) / cpw);
+  if (key == 4) hipLaunchKernelGGL((k_eig_nom<4, 1>), grid, dim3(64), 0, st, P, B);
+  else if (key == 8) hipLaunchKernelGGL((k_eig_nom<8, 1>), grid, dim3(64), 0, st, P, B);
+  else hipLaunchKernelGGL((k_eig_nom<EIG16_G, 16 / EIG16_G>), grid, dim3(64), 0, st, P, B);
   kmark("k_eig_nom", st);
 }
 
@@ -631,15 +654,15 @@ bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream
   return false;
   switch (kind) {
     case BK_EIG_LIN:
-      if (key == 4) hipLaunchKernelGGL(k_eig_lin_batch<4>, g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL(k_eig_lin_batch<8>, g, b, 0, st, T);
-      else if (key == 16) hipLaunchKernelGGL(k_eig_lin_batch<16>, g, b, 0, st, T);
+      if (key == 4) hipLaunchKernelGGL((k_eig_lin_batch<4, 1>), g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL((k_eig_lin_batch<8, 1>), g, b, 0, st, T);
+      else if (key == 16) hipLaunchKernelGGL((k_eig_lin_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
       else return false;
       return true;
     case BK_EIG_NOM:
-      if (key == 4) hipLaunchKernelGGL(k_eig_nom_batch<4>, g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL(k_eig_nom_batch<8>, g, b, 0, st, T);
-      else if (key == 16) hipLaunchKernelGGL(k_eig_nom_batch<16>, g, b, 0, st, T);
+      if (key == 4) hipLaunchKernelGGL((k_eig_nom_batch<4, 1>), g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL((k_eig_nom_batch<8, 1>), g, b, 0, st, T);
+      else if (key == 16) hipLaunchKernelGGL((k_eig_nom_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
       else return false;
       return true;
     case BK_COMPACT: hipLaunchKernelGGL(k_compact_batch, g, b, 0, st, T); return true;

@@ -50,6 +50,14 @@ struct StageBufs {
   float* d_atb;
 };
 
+// Lane layout of the eigen kernels.  Matrices of order <= 4, <= 8 and <= 16 (the kernel's key)
+// use 4, 8 and EIG16_G lanes per candidate; order-16 matrices put 16 / EIG16_G rows on a lane.
+// 16 lanes (one row each) measured faster than 8 lanes with two rows each, whose doubled LDS per
+// wave halves the waves in flight (c3 LIN_A eigensolve 7.8 vs 8.0 ms).
+constexpr int EIG16_G = 16;
+inline int eig_key(int order) { return order <= 4 ? 4 : (order <= 8 ? 8 : 16); }
+inline int eig_lanes(int key) { return key == 16 ? EIG16_G : key; }
+
 // Batched launches: one launch of a stage kernel covers a group of problems (same kernel
 // template instance).  Problem j of the group owns the group-local blocks [start[j], start[j+1]);
 // a workgroup maps its block to (problem, local block, the problem's block count) by binary search
@@ -64,8 +72,8 @@ struct BatchTab {
 };
 // Stage kernels that have a batched form (key: the template parameter of the instance).
 enum BatchKernel : int {
-  BK_EIG_LIN = 0,   // key G
-  BK_EIG_NOM,       // key G
+  BK_EIG_LIN = 0,   // key: matrix capacity (eig_key)
+  BK_EIG_NOM,       // key: matrix capacity (eig_key)
   BK_COMPACT,
   BK_FIT_FP64,      // key NB
   BK_SWEEP_FP64,    // key NS
