@@ -77,7 +77,7 @@ template <int G, int R>
 __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int N = G * R, S = N + 1, CPW = 64 / G;
   __shared__ double smat[CPW][N * S];
-  __shared__ double svec[CPW][5 * N];
+  __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
   const int c = bx * CPW + gi;
   if (c >= B.n) return;
@@ -85,8 +85,11 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
   double* V = svec[gi];
   double* WR = V + N;
   double* WI = V + 2 * N;
-  double* MAG = V + 3 * N;
-  double* IM = V + 4 * N;
+  // after the solve: |lambda| in V (the reduction's scratch), the continuous-time mode (re, im) of
+  // eigenvalue r in WR[r], WI[r] (each lane overwrites only the entries it has read)
+  double* MAG = V;
+  double* RE = WR;
+  double* IM = WI;
   const int ns = P->ns, nd = P->nd, nt = P->nt, n = B.n;
   const float* g = B.gains + (size_t)c * P->cuda_w;
   CE_ROWS(r) {
@@ -134,7 +137,7 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
       MAG[r] = mag;
       re[q_] = cedm::log(cedm::fmax_(mag, EPS30)) / P->ts;
       im[q_] = cedm::atan2(wi, wr) / P->ts;
-      V[r] = re[q_];
+      RE[r] = re[q_];
       IM[r] = im[q_];
     }
   }
@@ -145,7 +148,7 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
   CE_ROWS(r) {
     bool keep = false;
     if (r < nt) {
-      const double pre = (r == 0) ? BIG10 : V[r - 1];
+      const double pre = (r == 0) ? BIG10 : RE[r - 1];
       const double pim = (r == 0) ? BIG10 : IM[r - 1];
       const bool dup = cedm::fabs_(re[q_] - pre) < EPS6 && cedm::fabs_(cedm::fabs_(im[q_]) - cedm::fabs_(pim)) < EPS6;
       keep = !dup;
