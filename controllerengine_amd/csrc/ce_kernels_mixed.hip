@@ -56,24 +56,18 @@ __device__ __forceinline__ Cplx cexpm1(double ar, double ai) {
 // reverse.  Lane = list slot (coalesced reads of the SoA trajectory), 8 modes per lane
 // (blockIdx.y selects modes 8y..8y+7), blockIdx.z = time part.
 constexpr int ATB_MODES = 8;
-__device__ __forceinline__ void k_atb_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
-  const int idx = bx * 64 + threadIdx.x;
-  if (idx >= *B.count) return;
-  const int c = B.list[idx];
-  const double* row = B.info + (size_t)c * P->info_w;
-  if (B.flags[c] & F_UNSTABLE) return;
-  if (!(row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)))) return;
-  const int u = B.uniq[c];
-  const int e0 = by * ATB_MODES;
-  if (e0 >= u) return;
-  const int n = B.n, N = P->fit_count;
-  const int tb = bz * B.atb_chunk;
-  const int te = (tb + B.atb_chunk < N) ? tb + B.atb_chunk : N;
+
+// M modes of one lane's candidate over samples [tb, te), newest first; samples are loaded four at a
+// time ahead of their Horner steps.  Each mode's recurrence is the same operation sequence for
+// any M, so the sums do not depend on the mode count the wave runs with.
+template <int M>
+__device__ __forceinline__ void atb_run(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int idx, int u,
+                                        int e0, int tb, int te, int bz, double yss) {
+  const int n = B.n;
   const double ts = P->ts;
-  const double yss = row[P->info_freqresp + 3];
-  double zr[ATB_MODES], zi[ATB_MODES], ar[ATB_MODES], ai[ATB_MODES];
+  double zr[M], zi[M], ar[M], ai[M];
 #pragma unroll
-  for (int k = 0; k < ATB_MODES; k++) {
+  for (int k = 0; k < M; k++) {
     zr[k] = 0.0;
     zi[k] = 0.0;
     ar[k] = 0.0;
@@ -88,21 +82,51 @@ __device__ __forceinline__ void k_atb_body(const ProblemDev* __restrict__ P, con
     }
   }
   const float* ycol = B.ybuf32 + idx;
-#pragma unroll 4
-  for (int t = te - 1; t >= tb; t--) {
-    const double y = (double)ycol[(size_t)t * n] - yss;
+  auto step = [&](float yf) {
+    const double y = (double)yf - yss;
 #pragma unroll
-    for (int k = 0; k < ATB_MODES; k++) {
+    for (int k = 0; k < M; k++) {
       const double nr = __builtin_fma(ar[k], zr[k], __builtin_fma(-ai[k], zi[k], y));
       const double ni = __builtin_fma(ar[k], zi[k], ai[k] * zr[k]);
       ar[k] = nr;
       ai[k] = ni;
     }
+  };
+  int t = te - 1;
+  for (; t - 3 >= tb; t -= 4) {
+    const float y0 = ycol[(size_t)t * n], y1 = ycol[(size_t)(t - 1) * n];
+    const float y2 = ycol[(size_t)(t - 2) * n], y3 = ycol[(size_t)(t - 3) * n];
+    step(y0);
+    step(y1);
+    step(y2);
+    step(y3);
   }
+  for (; t >= tb; t--) step(ycol[(size_t)t * n]);
   double2* out = B.atbp + ((size_t)bz * P->nt + e0) * n + idx;
 #pragma unroll
-  for (int k = 0; k < ATB_MODES; k++)
+  for (int k = 0; k < M; k++)
     if (e0 + k < u) out[(size_t)k * n] = make_double2(ar[k], ai[k]);
+}
+
+__device__ __forceinline__ void k_atb_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int idx = bx * 64 + threadIdx.x;
+  if (idx >= *B.count) return;
+  const int c = B.list[idx];
+  const double* row = B.info + (size_t)c * P->info_w;
+  if (B.flags[c] & F_UNSTABLE) return;
+  if (!(row[P->info_scoring] == 0.0 || (B.diag && (B.d_ata || B.d_atb)))) return;
+  const int u = B.uniq[c];
+  const int e0 = by * ATB_MODES;
+  if (e0 >= u) return;
+  const int N = P->fit_count;
+  const int tb = bz * B.atb_chunk;
+  const int te = (tb + B.atb_chunk < N) ? tb + B.atb_chunk : N;
+  const double yss = row[P->info_freqresp + 3];
+  // the wave runs with the largest mode count among its active lanes (2, 4 or 8)
+  const int um = u - e0;
+  if (__ballot(um > 4) != 0ull) atb_run<8>(P, B, c, idx, u, e0, tb, te, bz, yss);
+  else if (__ballot(um > 2) != 0ull) atb_run<4>(P, B, c, idx, u, e0, tb, te, bz, yss);
+  else atb_run<2>(P, B, c, idx, u, e0, tb, te, bz, yss);
 }
 
 __global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, StageBufs B) {
@@ -171,30 +195,38 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
     const int e = g;
     const double re = B.mode_re[(size_t)e * n + c], im = B.mode_im[(size_t)e * n + c];
     const double sh = cedm::fmax_(re * t0, re * t1);
+    // basis value at each part's first sample: the window start's value advanced by
+    // z_e^chunk = exp((re + i im) chunk ts) per part
+    const double mag = cedm::exp(re * tau0 - sh);
+    double sw0, cw0;
+    cedm::sincos(im * tau0, sw0, cw0);
     double sr = 0.0, si = 0.0;
-    for (int q = 0; q < B.atb_parts; q++) {
-      const double tau = (double)(f0 + q * B.atb_chunk) * ts;
-      const double m0 = cedm::exp(re * tau - sh);
-      double s0, c0;
-      cedm::sincos(im * tau, s0, c0);
-      const double br = m0 * c0, bi = m0 * s0;
-      const double2 pv = B.atbp[((size_t)q * nt + e) * n + idx];
-      sr = sr + (br * pv.x - bi * pv.y);
-      si = si + (br * pv.y + bi * pv.x);
+    {
+      double br = mag * cw0, bi = mag * sw0;
+      const double tch = (double)B.atb_chunk * ts;
+      const double mw = cedm::exp(re * tch);
+      double sw, cw;
+      cedm::sincos(im * tch, sw, cw);
+      const double wr = mw * cw, wi = mw * sw;
+      for (int q = 0; q < B.atb_parts; q++) {
+        const double2 pv = B.atbp[((size_t)q * nt + e) * n + idx];
+        sr = sr + (br * pv.x - bi * pv.y);
+        si = si + (br * pv.y + bi * pv.x);
+        const double nr = br * wr - bi * wi;
+        bi = br * wi + bi * wr;
+        br = nr;
+      }
     }
     vec[slot][0][2 * e] = sr;
     vec[slot][0][2 * e + 1] = si;
     const double ar = re * ts, ai = im * ts;
     const Cplx e1 = cexpm1(ar, ai), en = cexpm1((double)N * ar, (double)N * ai);
-    const double mag = cedm::exp(re * tau0 - sh);
-    double sw, cw;
-    cedm::sincos(im * tau0, sw, cw);
     PM(0, e) = e1.r;
     PM(1, e) = e1.i;
     PM(2, e) = en.r;
     PM(3, e) = en.i;
-    PM(4, e) = mag * cw;
-    PM(5, e) = mag * sw;
+    PM(4, e) = mag * cw0;
+    PM(5, e) = mag * sw0;
     PM(6, e) = cedm::expm1(2.0 * ar);
     PM(7, e) = cedm::expm1(2.0 * ((double)N * ar));
   }
