@@ -353,8 +353,9 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   auto cdiv = [](int a, int c) { return (a + c - 1) / c; };
   using PP = ce_batch::Prob;
   group(BK_EIG_LIN, [&](const PP& p) { return eig_key(p.nt); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(p.nt)); }, "k_eig_lin");
-  group(BK_EIG_NOM, [&](const PP& p) { return eig_key(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(2 * p.ns)); }, "k_eig_nom");
   group(BK_COMPACT, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_compact");
+  // the nominal eigensolve walks each problem's compacted list, so it follows the compaction
+  group(BK_EIG_NOM, [&](const PP& p) { return eig_key(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(2 * p.ns)); }, "k_eig_nom");
   {  // simulation: one group per structure
     std::map<std::string, Launch> g;
     for (int j = 0; j < J; j++) {
@@ -492,7 +493,7 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
     Launch& L = b->launches[i];
     const int total = L.start.back();
     const bool on_side = fork && (L.kind == BK_EIG_NOM || L.kind == BK_SWEEP_MARK32 || L.kind == BK_REFINE64);
-    if (fork && !forked && L.kind != BK_EIG_LIN) {
+    if (fork && !forked && L.kind != BK_EIG_LIN && L.kind != BK_COMPACT) {
       BCHK(hipEventRecord(b->ev_fork, st));
       BCHK(hipStreamWaitEvent(b->side, b->ev_fork, 0));
       forked = true;

@@ -18,7 +18,6 @@
 
 namespace ce {
 // kernels (ce_kernels_fp64.hip, ce_kernels_mixed.hip)
-void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
@@ -80,7 +79,7 @@ struct ce_engine {
   JitCache* jit = nullptr;
   int last_sim_jit = 0;
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
-  // simulation and the fit (forked after the LIN_A eigensolve, joined before the crossing scan)
+  // simulation and the fit (forked after the compaction, joined before the crossing scan)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // start signal of the gated simulation (one int, value = gate_gen of the last gated launch)
@@ -258,9 +257,11 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     ensure(E->gate, sizeof(int));
     launch_fill_i32((int*)E->gate.p, 0, 1, st);
   }
-  if (fork) launch_eig_lin(H.nt, P, B, st);
-  else launch_eig_fp64(H.ns, H.nt, P, B, st);
+  launch_eig_lin(H.nt, P, B, st);
   launch_compact(P, B, st);
+  // the nominal eigensolve walks the compacted list: after the compaction, on this stream or the
+  // second one
+  if (!fork) launch_eig_nom(H.ns, P, B, st);
   if (fork) {
     HIPCHK(hipEventRecord(E->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));

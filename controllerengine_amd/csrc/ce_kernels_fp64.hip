@@ -191,15 +191,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 
-// nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79)
+// nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79), for the
+// candidates of the compacted list that are not already flagged
 template <int G, int R>
 __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int N = G * R, S = N + 1, CPW = 64 / G;
   __shared__ double smat[CPW][N * S];
   __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
-  const int c = bx * CPW + gi;
-  if (c >= B.n) return;
+  // candidates from the compacted list (after k_compact): the groups of a wave are all candidates
+  // that reach the simulation, most of which need the nominal check
+  const int idx = bx * CPW + gi;
+  if (idx >= *B.count) return;
+  const int c = B.list[idx];
   const int flags = B.flags[c];
   if (flags & F_NOMINAL_SKIPPED) return;
   double* T = smat[gi];
@@ -598,11 +602,6 @@ void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t
   else if (key == 8) hipLaunchKernelGGL((k_eig_nom<8, 1>), grid, dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL((k_eig_nom<EIG16_G, 16 / EIG16_G>), grid, dim3(64), 0, st, P, B);
   kmark("k_eig_nom", st);
-}
-
-void launch_eig_fp64(int ns, int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  launch_eig_lin(nt, P, B, st);
-  launch_eig_nom(ns, P, B, st);
 }
 
 void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
