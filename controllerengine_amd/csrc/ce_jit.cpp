@@ -105,18 +105,35 @@ struct Gen {
 // the source.  Structural form: only the dimensions and the instruction structure (opcodes,
 // columns, operand kinds) are fixed; constants and sizes are loaded from the problem descriptor
 // (Pp, byte offsets into ProblemDev), so one compiled kernel serves every problem of that structure.
-static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
+// Structural form, constants of the time step: with `hoist`, each is loaded once before the loop
+// into a named register value (kcN); otherwise they are read every step through Pq, which an empty
+// asm statement redefines (in scalar registers) every step, so the loads stay scalar loads from the
+// scalar cache instead of being hoisted into vector registers for the whole loop (for structures
+// with more constants than registers to spare).  *n_hoisted: 32-bit words of hoisted constants.
+static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, bool hoist = false,
+                     int* n_hoisted = nullptr) {
   const int ns = P.ns, nd = P.nd, nw = P.nw;
   std::ostringstream& o = g.o;
   const std::string T = g.T();
-  // constants inside the time loop are read through Pq, which an empty asm statement redefines
-  // (in scalar registers) every step: the loads stay scalar loads from the scalar cache instead of
-  // being hoisted into vector registers for the whole loop
   bool in_loop = false;
+  std::map<size_t, std::string> hname;
+  std::ostringstream hdecl;
+  int hwords = 0;
   // loads through the constant address space: read-only, so uniform addresses become scalar loads
-  auto ld = [&](const char* ty, size_t off) {
+  auto ld0 = [&](const char* ty, size_t off, const char* base) {
     return std::string("(*(const __attribute__((address_space(4))) ") + ty + "*)((const __attribute__((address_space(4))) char*)" +
-           (in_loop ? "Pq" : "Pp") + " + " + std::to_string(off) + "))";
+           base + " + " + std::to_string(off) + "))";
+  };
+  auto ld = [&](const char* ty, size_t off) {
+    if (!in_loop) return ld0(ty, off, "Pp");
+    if (!hoist) return ld0(ty, off, "Pq");
+    auto it = hname.find(off);
+    if (it != hname.end()) return it->second;
+    const std::string nm = "kc" + std::to_string(hname.size());
+    hdecl << "  const " << ty << " " << nm << " = " << ld0(ty, off, "Pp") << ";\n";
+    hwords += (std::string(ty) == "double") ? 2 : 1;
+    hname[off] = nm;
+    return nm;
   };
   // plant / observer coefficient: double field (FP64 form) or its float copy (mixed form)
   auto K = [&](double v, size_t offd, size_t offf) {
@@ -143,12 +160,14 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   o << "  " << T << "* yb = (" << T << "*)A.ybuf + idx;\n";
   o << "  float* dy = A.dy;\n";
   o << "  const " << T << " amp = " << K(P.ref_amp, CE_OFF(ref_amp), CE_OFF(fref_amp)) << ";\n";
-  if (structural) o << "  const char* Pq = Pp;\n";
+  if (structural && !hoist) o << "  const char* Pq = Pp;\n";
+  const std::string hoist_mark = "  /*hoisted constants*/\n";
+  if (structural && hoist) o << hoist_mark;
   // one time step; the phase flags (fit window stored, steady-state sum, reference still zero,
   // full-trajectory dump) are compile-time constants in the phase loops below, so a step carries
   // no per-step branches besides the loop's own
   o << "  auto step = [&](const int t, auto ST, auto SS, auto RZ, auto DY) __attribute__((always_inline)) {\n";
-  if (structural)
+  if (structural && !hoist)
     o << "    asm volatile(\"\" : \"+s\"(Pq));\n"
          "    Pq = (const char*)(((unsigned long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long)Pq >> 32)) << 32) |\n"
          "                       (unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)(unsigned long)Pq));\n";
@@ -245,6 +264,14 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural) {
   for (int i = 0; i < nd; i++) o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
   o << "  };\n";
   in_loop = false;
+  if (structural && hoist) {
+    std::string text = o.str();
+    const size_t at = text.find(hoist_mark);
+    text.replace(at, hoist_mark.size(), hdecl.str());
+    o.str(text);
+    o.seekp(0, std::ios_base::end);
+  }
+  if (n_hoisted) *n_hoisted = hwords;
   // the horizon in segments between the phase boundaries; within one the flags are constant
   o << "  const bool wdy = dy != nullptr;\n"
        "  int t = 0;\n"
@@ -334,7 +361,22 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
     << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
   o << kSimArgs;
   o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
-  gen_body(g, P, mixed, true);
+  {
+    // constants hoisted out of the time loop when they fit in a modest register budget
+    constexpr int kHoistMaxWords = 64;
+    Gen gh;
+    gh.f32 = mixed;
+    int words = 0;
+    gen_body(gh, P, mixed, true, true, &words);
+    if (words <= kHoistMaxWords) {
+      o << gh.o.str();
+    } else {
+      Gen gr;
+      gr.f32 = mixed;
+      gen_body(gr, P, mixed, true, false);
+      o << gr.o.str();
+    }
+  }
   o << "}\n";
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
