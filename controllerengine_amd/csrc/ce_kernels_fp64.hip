@@ -309,9 +309,8 @@ __device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P
     while (p >= rb) { p -= rb; rb++; }
     cb = p;
   }
-  // basis evaluations of a TB-sample block: NM modes x TB samples spread over the G lanes; the
-  // candidate's mode table (decay rate, frequency, shift) is staged in LDS
-  constexpr int EVALS = NM * TB / G;
+  // basis evaluations of a TB-sample block spread over the G lanes; the candidate's mode table
+  // (decay rate, frequency, shift) is staged in LDS
   const double ts = P->ts, t0 = P->fit_t0, t1 = P->fit_t1;
   for (int e = g; e < NM; e += G) {
     double re = 0.0, im = 0.0, sh = 0.0;
@@ -342,14 +341,18 @@ __device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P
   const double* ycol = B.ybuf + (active ? idx : 0);
   const int rbase = rb * 8, cbase = cb * 8 + coff;
   const int nc = 2 * u;
+  // the columns of modes u..NM-1 stay zero; each round evaluates only the u x TB basis values the
+  // candidate has, spread over the group's lanes (same values, fewer idle evaluations)
+  for (int k = g; k < TB * NCP; k += G) phi[slot][k / NCP][k % NCP] = 0.0;
+  const int ue = active ? u : 0;
+  __syncthreads();
   for (int tb = 0; tb < cnt; tb += TB) {
 #pragma unroll 1
-    for (int q = 0; q < EVALS; q++) {
-      const int k = g + q * G;
-      const int e = k % NM, s = k / NM;
+    for (int k = g; k < ue * TB; k += G) {
+      const int e = k % ue, s = k / ue;
       const int t = tb + s;
       double cv = 0.0, sv = 0.0;
-      if (active && e < u && t < cnt) {
+      if (t < cnt) {
         const double tval = (double)(f0 + t) * ts;
         const double decay = cedm::exp(cedm::fma_(modes[slot][e][0], tval, -modes[slot][e][2]));
         double sn, cs;
