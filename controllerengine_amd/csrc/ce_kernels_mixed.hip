@@ -57,9 +57,9 @@ __device__ __forceinline__ Cplx cexpm1(double ar, double ai) {
 // (blockIdx.y selects modes 8y..8y+7), blockIdx.z = time part.
 constexpr int ATB_MODES = 8;
 
-// M modes of one lane's candidate over samples [tb, te), newest first; samples are loaded four at a
-// time ahead of their Horner steps.  Each mode's recurrence is the same operation sequence for
-// any M, so the sums do not depend on the mode count the wave runs with.
+// M modes of one lane's candidate over samples [tb, te), newest first; samples are loaded in
+// groups, one group ahead of the Horner steps.  Each mode's recurrence is the same operation
+// sequence for any M, so the sums do not depend on the mode count the wave runs with.
 template <int M>
 __device__ __forceinline__ void atb_run(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int idx, int u,
                                         int e0, int tb, int te, int bz, double yss) {
@@ -92,14 +92,23 @@ __device__ __forceinline__ void atb_run(const ProblemDev* __restrict__ P, const 
       ai[k] = ni;
     }
   };
+  // groups of 8 samples; the next group's loads are issued before the current group's steps
+  constexpr int GS = 8;
   int t = te - 1;
-  for (; t - 3 >= tb; t -= 4) {
-    const float y0 = ycol[(size_t)t * n], y1 = ycol[(size_t)(t - 1) * n];
-    const float y2 = ycol[(size_t)(t - 2) * n], y3 = ycol[(size_t)(t - 3) * n];
-    step(y0);
-    step(y1);
-    step(y2);
-    step(y3);
+  float cur[GS], nxt[GS];
+  if (t - (GS - 1) >= tb) {
+#pragma unroll
+    for (int j = 0; j < GS; j++) nxt[j] = ycol[(size_t)(t - j) * n];
+  }
+  for (; t - (GS - 1) >= tb; t -= GS) {
+#pragma unroll
+    for (int j = 0; j < GS; j++) cur[j] = nxt[j];
+    if (t - GS - (GS - 1) >= tb) {
+#pragma unroll
+      for (int j = 0; j < GS; j++) nxt[j] = ycol[(size_t)(t - GS - j) * n];
+    }
+#pragma unroll
+    for (int j = 0; j < GS; j++) step(cur[j]);
   }
   for (; t >= tb; t--) step(ycol[(size_t)t * n]);
   double2* out = B.atbp + ((size_t)bz * P->nt + e0) * n + idx;
