@@ -592,15 +592,30 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
     const float* mr = B.mr32 + (size_t)c * F * 3;
     const double* m64 = B.mag_raw + (size_t)c * F * 3;
     const int8_t* rf = B.refined + (size_t)c * F;
+    // the lane's float values and refinement marks are loaded together, ahead of their use
+    float fv[KMAX][3];
+    int rv[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      rv[k] = 0;
+      fv[k][0] = fv[k][1] = fv[k][2] = 0.0f;
+      if (k < nk) {
+        const int w = w0 + k;
+        rv[k] = rf[w];
+        fv[k][0] = mr[3 * w];
+        fv[k][1] = mr[3 * w + 1];
+        fv[k][2] = mr[3 * w + 2];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
       m[k] = raw[k] = bb[k] = 0.0;
       if (k >= nk) continue;
       const int w = w0 + k;
-      const int r = rf[w];
+      const int r = rv[k];
       if (r == 3) failp = true;
       if (r == 2) { m[k] = m64[3 * w]; raw[k] = m64[3 * w + 1]; bb[k] = m64[3 * w + 2]; }
-      else { m[k] = (double)mr[3 * w]; raw[k] = (double)mr[3 * w + 1]; bb[k] = (double)mr[3 * w + 2]; }
+      else { m[k] = (double)fv[k][0]; raw[k] = (double)fv[k][1]; bb[k] = (double)fv[k][2]; }
       if (!(cedm::isfinite_(m[k]) && cedm::isfinite_(raw[k]))) failp = true;
     }
   }

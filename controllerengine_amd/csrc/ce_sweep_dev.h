@@ -166,7 +166,9 @@ __device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int
 // above for parity).  A1 = Q H Q^T by Householder reflectors, so
 //   G(z) = K^T (zI - A1)^-1 L = (Q^T K)^T (zI - H)^-1 (Q^T L),
 // and each frequency costs an O(NS^2) solve of the Hessenberg system (partial pivoting between
-// the two candidate rows of each column) instead of an O(NS^3) dense elimination.
+// the two candidate rows of each column) instead of an O(NS^3) dense elimination.  The pivots are
+// inverted with the hardware reciprocal (1 ulp): these float values only locate the points that
+// are recomputed in FP64, with guard bands far wider than the rounding.
 template <int NS>
 __device__ __forceinline__ void hess_prep(const double* __restrict__ oa, const float* __restrict__ gg,
                                           float (&H)[NS][NS], float (&kq)[NS], float (&lq)[NS]) {
@@ -263,7 +265,7 @@ __device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, con
       vi[k + 1] = sw ? ai : bi;
     }
     const float pr = Mr[k][k], pi = Mi[k][k];
-    const float inv = 1.0f / (pr * pr + pi * pi);
+    const float inv = __builtin_amdgcn_rcpf(pr * pr + pi * pi);
     const float lr = (Mr[k + 1][k] * pr + Mi[k + 1][k] * pi) * inv;
     const float li = (Mi[k + 1][k] * pr - Mr[k + 1][k] * pi) * inv;
 #pragma unroll
@@ -286,7 +288,7 @@ __device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, con
       si = si - (Mr[i][j] * xi[j] + Mi[i][j] * xr[j]);
     }
     const float pr = Mr[i][i], pi = Mi[i][i];
-    const float inv = 1.0f / (pr * pr + pi * pi);
+    const float inv = __builtin_amdgcn_rcpf(pr * pr + pi * pi);
     xr[i] = (sr * pr + si * pi) * inv;
     xi[i] = (si * pr - sr * pi) * inv;
     fin = fin && __builtin_isfinite(xr[i]) && __builtin_isfinite(xi[i]);
