@@ -254,8 +254,9 @@ def main():
         print(json.dumps({"copy_bytes": src.numel() * 4}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: the candidate chain has no cross-GPU step in this build "
-              "(multi-GPU sharding would split candidate ranges); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the candidate chain has no cross-GPU step "
+              "(a multi-GPU run would split the candidate ranges across GPUs); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.config == "c4":
