@@ -116,6 +116,7 @@ struct ce_batch {
   int32_t* sfail = nullptr;
   int32_t* counts = nullptr;   // count per problem
   int32_t* rcounts = nullptr;  // refinement count per problem
+  int32_t* fcounts = nullptr;  // FP64 sweep list length per problem
   std::vector<Launch> launches;
   const float* last_gains = nullptr;
   // graph
@@ -257,8 +258,9 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   };
   const Seg s_info = seg(io * sizeof(double));
   const Seg s_bmin = seg(no * 8), s_sfail = seg(no * 4), s_counts = seg(J * 4), s_rcounts = seg(J * 4);
+  const Seg s_fcounts = seg(J * 4);
   struct PS {
-    Seg flags, uniq, mre, mim, oa, y, mag, list, mr32, refined, rlist, atbp;
+    Seg flags, uniq, mre, mim, oa, y, mag, list, mr32, refined, rlist, flist, atbp;
     int parts, chunk;
   };
   std::vector<PS> ps(J);
@@ -275,9 +277,10 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     s.mag = seg(n * (size_t)p.F * (mixed ? 3 : 2) * 8);
     s.list = seg(n * 4);
     if (mixed) {
-      s.mr32 = seg(n * (size_t)p.F * 3 * 4);
+      s.mr32 = seg(n * (size_t)p.F * 4 * 4);
       s.refined = seg(((n * (size_t)p.F + 3) / 4) * 4);
       s.rlist = seg(n * RMAX * sizeof(int2));
+      s.flist = seg(n * 4);
       int parts = std::max(1, std::min(parts_all, p.fc / 64));
       const int chunk = std::max(1, (p.fc + parts - 1) / parts);
       parts = std::max(1, (p.fc + chunk - 1) / chunk);
@@ -292,6 +295,7 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   b->sfail = (int32_t*)(W + s_sfail.off);
   b->counts = (int32_t*)(W + s_counts.off);
   b->rcounts = (int32_t*)(W + s_rcounts.off);
+  b->fcounts = (int32_t*)(W + s_fcounts.off);
   b->hb.assign(J, StageBufs{});
   b->hsargs.assign(J, SimArgs{});
   for (int j = 0; j < J; j++) {
@@ -319,6 +323,8 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
       B.refined = (int8_t*)(W + s.refined.off);
       B.rlist = (int2*)(W + s.rlist.off);
       B.rcount = b->rcounts + j;
+      B.flist = (int32_t*)(W + s.flist.off);
+      B.fcount = b->fcounts + j;
       B.atbp = (double2*)(W + s.atbp.off);
       B.atb_parts = s.parts;
       B.atb_chunk = s.chunk;
@@ -385,6 +391,7 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     group(BK_FITCF, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fitcf");
     group(BK_SWEEP_MARK32, [](const PP& p) { return p.ns; }, [](const PP& p, const PS&) { return p.n; }, "k_sweep_mark32");
     group(BK_REFINE64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_refine64");
+    group(BK_FULL64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_full64");
     group(BK_SCAN32, [](const PP&) { return 0; }, [](const PP& p, const PS&) { return p.n; }, "k_scan32");
   } else {
     group(BK_FIT_FP64, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fit_fp64");
@@ -475,7 +482,10 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   launch_fill_u64(b->bmin, 0x7ff0000000000000ull, b->total_n, st);
   launch_fill_i32(b->sfail, 0, b->total_n, st);
   launch_fill_i32(b->counts, 0, J, st);
-  if (mixed) launch_fill_i32(b->rcounts, 0, J, st);
+  if (mixed) {
+    launch_fill_i32(b->rcounts, 0, J, st);
+    launch_fill_i32(b->fcounts, 0, J, st);
+  }
   launch_info_init(d_in, b->info64, b->total_info, st);
   batch_mark(b, "fill", st);
   const ProblemDev* const* probs = (const ProblemDev* const*)b->ptrs.p;
@@ -492,7 +502,8 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   for (size_t i = 0; i < b->launches.size(); i++) {
     Launch& L = b->launches[i];
     const int total = L.start.back();
-    const bool on_side = fork && (L.kind == BK_EIG_NOM || L.kind == BK_SWEEP_MARK32 || L.kind == BK_REFINE64);
+    const bool on_side = fork && (L.kind == BK_EIG_NOM || L.kind == BK_SWEEP_MARK32 || L.kind == BK_REFINE64 ||
+                                  L.kind == BK_FULL64);
     if (fork && !forked && L.kind != BK_EIG_LIN && L.kind != BK_COMPACT) {
       BCHK(hipEventRecord(b->ev_fork, st));
       BCHK(hipStreamWaitEvent(b->side, b->ev_fork, 0));

@@ -62,7 +62,7 @@ struct ce_engine {
     size_t cap = 0;
   };
   Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
-  Buf mr32, refined, rlist, rcount, atbp;
+  Buf mr32, refined, rlist, rcount, flist, fcount, atbp;
   // per-kernel timing (ce_engine_set_profiling)
   int profiling = 0;
   std::vector<hipEvent_t> mark_ev;
@@ -195,10 +195,12 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   B.list = (int32_t*)ensure(E->list, nn * sizeof(int32_t));
   B.count = (int32_t*)ensure(E->count, sizeof(int32_t));
   if (mixed) {
-    B.mr32 = (float*)ensure(E->mr32, nn * (size_t)F * 3 * sizeof(float));
+    B.mr32 = (float*)ensure(E->mr32, nn * (size_t)F * 4 * sizeof(float));
     B.refined = (int8_t*)ensure(E->refined, ((nn * (size_t)F + 3) / 4) * 4);
     B.rlist = (int2*)ensure(E->rlist, nn * RMAX * sizeof(int2));
     B.rcount = (int32_t*)ensure(E->rcount, sizeof(int32_t));
+    B.flist = (int32_t*)ensure(E->flist, nn * sizeof(int32_t));
+    B.fcount = (int32_t*)ensure(E->fcount, sizeof(int32_t));
     // time parts of the right-hand-side pass: enough waves to fill the chip, >= 64 samples each
     const int mb = (nt + 7) / 8;
     const long long waves = (long long)mb * (long long)((nn + 63) / 64);
@@ -222,6 +224,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   launch_fill_i32(B.count, 0, 1, st);
   if (mixed) {
     launch_fill_i32(B.rcount, 0, 1, st);
+    launch_fill_i32(B.fcount, 0, 1, st);
   }
   launch_info_init(d_info_in, B.info, nn * H.info_w, st);
   E->last_stream = st;
@@ -332,7 +335,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
                   &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
-                  &E->rcount, &E->atbp, &E->gate})
+                  &E->rcount, &E->flist, &E->fcount, &E->atbp, &E->gate})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

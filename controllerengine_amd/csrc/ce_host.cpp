@@ -404,7 +404,7 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   const int F = c.freq;
   const size_t head = (sizeof(ProblemDev) + 255) & ~(size_t)255;
   const size_t tab = ((size_t)F * 8 + 255) & ~(size_t)255;
-  blob.assign(head + 7 * tab, 0);
+  blob.assign(head + 8 * tab, 0);
   ProblemDev& P = *reinterpret_cast<ProblemDev*>(blob.data());
   P.ns = c.statespace_dim; P.nd = c.discrete_dim; P.nt = P.ns + P.nd; P.nw = c.nonlin_dim;
   P.T = c.time; P.F = F;
@@ -460,6 +460,7 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   P.off_pi = P.off_pr + (int64_t)tab;
   P.off_pabs = P.off_pi + (int64_t)tab;
   P.off_pok = P.off_pabs + (int64_t)tab;
+  P.off_zm1 = P.off_pok + (int64_t)tab;
   P.bytes_total = (int64_t)blob.size();
   double* zr = reinterpret_cast<double*>(blob.data() + P.off_zr);
   double* zi = reinterpret_cast<double*>(blob.data() + P.off_zi);
@@ -468,6 +469,7 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   double* pim = reinterpret_cast<double*>(blob.data() + P.off_pi);
   double* pab = reinterpret_cast<double*>(blob.data() + P.off_pabs);
   int32_t* pok = reinterpret_cast<int32_t*>(blob.data() + P.off_pok);
+  double* zm1 = reinterpret_cast<double*>(blob.data() + P.off_zm1);
   // log-spaced grid from freq_min to Nyquist (reference src/main.cu:421-428), evaluated in double
   const double lmin = cedm::log((double)c.freq_min), lmax = cedm::log((double)d.freq_max);
   const double inc = (lmax - lmin) / (double)(F - 1);
@@ -477,6 +479,10 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
     cedm::sincos(theta, s, co);
     zr[w] = co;
     zi[w] = s;
+    // Re(z) - 1 = -2 sin^2(theta / 2), without the cancellation of co - 1 at low frequencies
+    double sh, chh;
+    cedm::sincos(0.5 * theta, sh, chh);
+    zm1[w] = -2.0 * sh * sh;
     th[w] = cedm::atan2(s, co);
     double xr[MAXS], xi[MAXS];
     const bool ok = shifted_solve_host(P.pa, ns, co, s, P.pb, xr, xi);

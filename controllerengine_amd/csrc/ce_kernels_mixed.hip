@@ -31,6 +31,7 @@ namespace ce {
 constexpr float GUARD_MAG = 1e-3f;   // |m - 1| below this: refine in FP64
 constexpr float GUARD_PHASE = 1e-3f; // phase within this of an odd multiple of pi, or a jump within
                                      // this of pi: refine in FP64
+constexpr float BAND_SPREAD = 1e-6f; // b band over a flat minimum: refine only its smallest point
 
 // ------------------------------------------------------------------------------------------------
 struct Cplx {
@@ -314,10 +315,10 @@ __global__ __launch_bounds__(64) void k_fitcf_batch(BatchTab T) {
 // passes with a wave prefix over the lanes' chunk totals.
 constexpr int KMAX = 8;  // F <= 512
 
-__device__ __forceinline__ bool near_odd_pi(float ph) {
+__device__ __forceinline__ bool near_odd_pi(float ph, float tol) {
   const float x = ph + (float)PI_D;
   const float k = __builtin_rintf(x / (float)TWO_PI_D);
-  return __builtin_fabsf(x - k * (float)TWO_PI_D) < GUARD_PHASE;
+  return __builtin_fabsf(x - k * (float)TWO_PI_D) < tol;
 }
 
 template <typename V>
@@ -331,10 +332,20 @@ __device__ __forceinline__ V wave_excl_sum(V v, int lane) {
   return incl - v;
 }
 
+// Candidate c's float sweep is not trusted: append it to the FP64 list (one entry per candidate).
+__device__ __forceinline__ void sweep_fallback(const StageBufs& B, int c, int lane) {
+  if (lane == 0) {
+    const int pos = atomicAdd(B.fcount, 1);
+    B.flist[pos] = c;
+    atomicOr(&B.flags[c], F_SWEEP_FP64);
+  }
+}
+
 // Float sweep of every frequency, then the points whose float values show (or come within a guard
 // band of) a gain or phase crossing, a non-finite value, or the smallest b(w) (and those within a
-// band of it) are marked for FP64 refinement and appended to the dense refinement list, at most
-// RMAX per candidate (crossing points first).
+// band of it) are marked for FP64 refinement and appended to the dense refinement list.  The guard
+// bands are widened by each point's float error bound.  A candidate with more than RMAX marked
+// points has its whole sweep recomputed in FP64 instead.
 template <int NS>
 __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int c = bx;
@@ -346,49 +357,53 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   const double* oa = B.open_a + (size_t)c * NS * NS;
   const float* gg = B.gains + (size_t)c * P->cuda_w;
   bool failp = false;
-  float* mr = B.mr32 + (size_t)c * F * 3;
+  float* mr = B.mr32 + (size_t)c * F * 4;
   const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
   // the sweep points go to memory and are read back below, so that nothing else is live across
   // the register-heavy elimination
-  float H[NS][NS], kq[NS], lq[NS];
-  hess_prep<NS>(oa, gg, H, kq, lq);
+  double H[NS][NS], kq[NS], lq[NS], hnorm;
+  hess_prep<NS>(oa, gg, H, kq, lq, hnorm);
 #pragma unroll 1
   for (int k = 0; k < nk; k++) {
     const int w = w0 + k;
-    float mm = __builtin_nanf(""), rr = __builtin_nanf(""), b2 = __builtin_nanf("");
+    float mm = __builtin_nanf(""), rr = __builtin_nanf(""), b2 = __builtin_nanf(""), e2 = 1.0f;
     if (!pok[w]) {
       failp = true;
     } else {
-      float m1, r1, bq;
-      if (hess_point<NS>(P, H, kq, lq, w, m1, r1, bq)) { mm = m1; rr = r1; b2 = bq; }
+      float m1, r1, bq, e1;
+      if (hess_point<NS>(P, H, kq, lq, hnorm, w, m1, r1, bq, e1)) { mm = m1; rr = r1; b2 = bq; e2 = e1; }
     }
-    mr[3 * w] = mm;
-    mr[3 * w + 1] = rr;
-    mr[3 * w + 2] = b2;
+    mr[4 * w] = mm;
+    mr[4 * w + 1] = rr;
+    mr[4 * w + 2] = b2;
+    mr[4 * w + 3] = e2;
   }
   if (__ballot(failp)) {
     if (lane == 0) B.sweep_fail[c] = 1;
     return;
   }
-  float m[KMAX], raw[KMAX], bb[KMAX];
+  float m[KMAX], raw[KMAX], bb[KMAX], er[KMAX];
 #pragma unroll
   for (int k = 0; k < KMAX; k++) {
     m[k] = raw[k] = bb[k] = __builtin_nanf("");
+    er[k] = 1.0f;
     if (k < nk) {
       const int w = w0 + k;
-      m[k] = mr[3 * w];
-      raw[k] = mr[3 * w + 1];
-      bb[k] = mr[3 * w + 2];
+      m[k] = mr[4 * w];
+      raw[k] = mr[4 * w + 1];
+      bb[k] = mr[4 * w + 2];
+      er[k] = mr[4 * w + 3];
     }
   }
   const float PIf = (float)PI_D, TPf = (float)TWO_PI_D;
   // unwrapped float phase: ph[w] = raw[0] + sum_{v=1..w} wrap(raw[v] - raw[v-1])
-  float lastRaw = raw[0], lastM = m[0];
+  float lastRaw = raw[0], lastM = m[0], lastEr = er[0];
 #pragma unroll
   for (int k = 1; k < KMAX; k++)
-    if (k < nk) { lastRaw = raw[k]; lastM = m[k]; }
+    if (k < nk) { lastRaw = raw[k]; lastM = m[k]; lastEr = er[k]; }
   const float prevRaw0 = __shfl_up(lastRaw, 1);
   const float prevM0 = __shfl_up(lastM, 1);
+  const float prevEr0 = __shfl_up(lastEr, 1);
   float dl[KMAX];
   float loc = 0.0f;
 #pragma unroll
@@ -416,7 +431,8 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   for (int k = 1; k < KMAX; k++)
     if (k < nk) lastPh = ph[k];
   const float prevPh0 = __shfl_up(lastPh, 1);
-  // crossing marks: interval (w-1, w) flagged -> mark both ends
+  // crossing marks: interval (w-1, w) flagged -> mark both ends.  Each guard is widened by the
+  // float error bound of the points involved (relative in |loop|, radians in the phase).
   unsigned cross = 0;
   bool send_prev = false;
 #pragma unroll
@@ -431,13 +447,15 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
     const float pm_ = (k == 0) ? prevM0 : m[k > 0 ? k - 1 : 0];
     const float pr_ = (k == 0) ? prevRaw0 : raw[k > 0 ? k - 1 : 0];
     const float pp_ = (k == 0) ? prevPh0 : ph[k > 0 ? k - 1 : 0];
+    const float pe_ = (k == 0) ? prevEr0 : er[k > 0 ? k - 1 : 0];
     const float delta = raw[k] - pr_;
-    const bool amb = __builtin_fabsf(__builtin_fabsf(delta) - PIf) < GUARD_PHASE;
+    const bool amb = __builtin_fabsf(__builtin_fabsf(delta) - PIf) < GUARD_PHASE + pe_ + er[k];
     const bool gcross = (pm_ - 1.0f) * (m[k] - 1.0f) <= 0.0f;
-    const bool gnear = __builtin_fabsf(pm_ - 1.0f) < GUARD_MAG || __builtin_fabsf(m[k] - 1.0f) < GUARD_MAG;
+    const bool gnear = __builtin_fabsf(pm_ - 1.0f) < GUARD_MAG + pe_ * pm_ ||
+                       __builtin_fabsf(m[k] - 1.0f) < GUARD_MAG + er[k] * m[k];
     const float ka = __builtin_floorf((pp_ + PIf) / TPf), kb = __builtin_floorf((ph[k] + PIf) / TPf);
     const bool pcross = ka != kb;
-    const bool pnear = near_odd_pi(pp_) || near_odd_pi(ph[k]);
+    const bool pnear = near_odd_pi(pp_, GUARD_PHASE + pe_) || near_odd_pi(ph[k], GUARD_PHASE + er[k]);
     if (gcross || gnear || pcross || pnear || amb || !fin) {
       cross |= 1u << k;
       if (k == 0) send_prev = true;
@@ -458,42 +476,52 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
     const int oi = __shfl_xor(bi, d);
     if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
   }
+  // band: every point whose lower bound on b reaches the smallest upper bound.  With the loop
+  // value L = P G off by a relative e, |1 + L| moves by at most e |L| and each square root of the
+  // denominator by a relative e, and the denominator is at least 1 + |L| (Cauchy-Schwarz), so
+  // |db| <= e (b + |L| / (1 + |L|)).  A point outside the band cannot hold the smallest b, and its
+  // float value cannot undercut it; the band (with the smallest point) is refined in FP64, so the
+  // minimum is the FP64 one.
+  float eb[KMAX];
+  float hi = __builtin_inff(), lo = __builtin_inff();
+#pragma unroll
+  for (int k = 0; k < KMAX; k++) {
+    eb[k] = er[k] * (bb[k] + m[k] / (1.0f + m[k]));
+    if (k < nk && bb[k] + eb[k] < hi) hi = bb[k] + eb[k];
+    if (k < nk && bb[k] - eb[k] < lo) lo = bb[k] - eb[k];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    hi = __builtin_fminf(hi, __shfl_xor(hi, d));
+    lo = __builtin_fminf(lo, __shfl_xor(lo, d));
+  }
   unsigned band = 0;
   if (bi != 0x7fffffff) {
-    const float lim = bv + (1e-3f + 1e-2f * bv);
+    // margin for the rounding of b itself from L (a few float ulps)
+    const float lim = hi * (1.0f + 1e-5f);
 #pragma unroll
     for (int k = 0; k < KMAX; k++)
-      if (k < nk && bb[k] <= lim) band |= 1u << k;
+      if (k < nk && bb[k] - eb[k] <= lim) band |= 1u << k;
     if (bi >= w0 && bi < w0 + nk) cross |= 1u << (bi - w0);
   }
-  band &= ~cross;
-  // keep at most RMAX marks: crossing points first, then band points, each in frequency order
-  const int nc1 = __popc(cross);
-  const int pre1 = wave_excl_sum(nc1, lane);
-  const int tot1 = __shfl(pre1 + nc1, 63);
-  unsigned keep = 0;
-  int kept = 0;
-#pragma unroll
-  for (int k = 0; k < KMAX; k++)
-    if ((cross >> k) & 1u) {
-      if (pre1 + kept < RMAX) keep |= 1u << k;
-      kept++;
-    }
-  const int budget = RMAX - tot1;
-  const int nc2 = __popc(band);
-  const int pre2 = wave_excl_sum(nc2, lane);
-  int kept2 = 0;
-#pragma unroll
-  for (int k = 0; k < KMAX; k++)
-    if ((band >> k) & 1u) {
-      if (pre2 + kept2 < budget) keep |= 1u << k;
-      kept2++;
-    }
-  if (lane == 0 && tot1 > RMAX) atomicOr(&B.flags[c], F_REFINE_OVERFLOW);
+  // More than RMAX marks: a band over a flat minimum (all its points within BAND_SPREAD of one
+  // another, so any of them is the minimum to within that) is dropped but for the smallest float
+  // point; otherwise the whole sweep is recomputed in FP64.
+  unsigned keep = cross | band;
+  int nkeep = __popc(keep);
+  int pre = wave_excl_sum(nkeep, lane);
+  int tot = __shfl(pre + nkeep, 63);
+  if (tot > RMAX && hi - lo <= BAND_SPREAD) {
+    keep = cross;
+    nkeep = __popc(keep);
+    pre = wave_excl_sum(nkeep, lane);
+    tot = __shfl(pre + nkeep, 63);
+  }
+  if (tot > RMAX) {
+    sweep_fallback(B, c, lane);
+    return;
+  }
   // refinement list (one atomic per wave) and the per-point state
-  const int nkeep = __popc(keep);
-  const int pre = wave_excl_sum(nkeep, lane);
-  const int tot = __shfl(pre + nkeep, 63);
   int base = 0;
   if (lane == 0 && tot > 0) base = atomicAdd(B.rcount, tot);
   base = __shfl(base, 0);
@@ -536,6 +564,41 @@ __device__ __forceinline__ void k_refine64_body(const ProblemDev* __restrict__ P
     out[1] = raw;
     out[2] = b;
     B.refined[(size_t)c * F + w] = ok ? 2 : 3;
+  }
+}
+
+// Every frequency of the listed candidates in FP64 (the FP64 chain's sweep point), one wave per
+// candidate, lanes over frequencies.
+template <int NS>
+__device__ __forceinline__ void k_full64_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+  const int total = *B.fcount;
+  const int F = P->F;
+  for (int i = bx; i < total; i += nbx) {
+    const int c = B.flist[i];
+    const double* oa = B.open_a + (size_t)c * NS * NS;
+    const float* gg = B.gains + (size_t)c * P->cuda_w;
+    for (int w = threadIdx.x; w < F; w += 64) {
+      double m, raw, b;
+      const bool ok = sweep_point<NS, double>(P, oa, gg, w, m, raw, b);
+      double* out = B.mag_raw + ((size_t)c * F + w) * 3;
+      out[0] = m;
+      out[1] = raw;
+      out[2] = b;
+      B.refined[(size_t)c * F + w] = ok ? 2 : 3;
+    }
+  }
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_full64(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_full64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_full64_batch(BatchTab T) {
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    k_full64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
 }
 
@@ -589,7 +652,7 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
   double m[KMAX], raw[KMAX], bb[KMAX];
   bool failp = false;
   if (!fail) {
-    const float* mr = B.mr32 + (size_t)c * F * 3;
+    const float* mr = B.mr32 + (size_t)c * F * 4;
     const double* m64 = B.mag_raw + (size_t)c * F * 3;
     const int8_t* rf = B.refined + (size_t)c * F;
     // the lane's float values and refinement marks are loaded together, ahead of their use
@@ -602,9 +665,9 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
       if (k < nk) {
         const int w = w0 + k;
         rv[k] = rf[w];
-        fv[k][0] = mr[3 * w];
-        fv[k][1] = mr[3 * w + 1];
-        fv[k][2] = mr[3 * w + 2];
+        fv[k][0] = mr[4 * w];
+        fv[k][1] = mr[4 * w + 1];
+        fv[k][2] = mr[4 * w + 2];
       }
     }
 #pragma unroll
@@ -754,6 +817,8 @@ static void launch_mark_refine_t(const ProblemDev* P, const StageBufs& B, hipStr
   kmark("k_sweep_mark32", st);
   hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
   kmark("k_refine64", st);
+  hipLaunchKernelGGL(k_full64<NS>, dim3(256), dim3(64), 0, st, P, B);
+  kmark("k_full64", st);
 }
 
 // float sweep + marking + FP64 refinement (independent of the simulation and the fit)
@@ -820,6 +885,7 @@ bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStrea
       return false;
     case BK_SWEEP_MARK32: CE_NS_CASES(k_sweep_mark32_batch)
     case BK_REFINE64: CE_NS_CASES(k_refine64_batch)
+    case BK_FULL64: CE_NS_CASES(k_full64_batch)
     case BK_SCAN32: hipLaunchKernelGGL(k_scan32_batch, g, b, 0, st, T); return true;
   }
   return false;

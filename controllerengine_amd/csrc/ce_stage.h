@@ -11,10 +11,17 @@ enum : int32_t {
   F_UNSTABLE = 1,          // some closed-loop eigenvalue has |lambda| > 1 (reference src/modalsetup.cu:26)
   F_NOMINAL_UNSTABLE = 2,  // nominal loop of the margins stage unstable (reference src/allmargin.cu:61-87)
   F_NOMINAL_SKIPPED = 4,   // nominal check not run (candidate already at or above unstable_err)
-  F_REFINE_OVERFLOW = 8    // mixed sweep: more than RMAX points wanted FP64 refinement
+  F_SWEEP_FP64 = 8         // mixed sweep: every point recomputed in FP64 (more than RMAX points
+                           // wanting refinement)
 };
 
 constexpr int RMAX = 64;   // FP64 refinement points per candidate in the mixed sweep
+// Mixed sweep: factor on the first-order error estimate of a point's FP64 Hessenberg solve
+// (rounding unit times the pivot conditioning times the cancellation of the closing dot product),
+// and the allowance for the float rounding of the loop value and its float tail.  The guard bands
+// of each point are widened by the sum.
+constexpr float SWEEP_ERR_SAFETY = 8.0f;
+constexpr float SWEEP_ERR_FLOOR = 1e-6f;
 
 struct StageBufs {
   int32_t n;                    // candidates in this launch
@@ -34,10 +41,12 @@ struct StageBufs {
   int32_t* list;                // n: candidates that reach the simulation (compacted)
   int32_t* count;               // 1: entries in list
   // mixed sweep
-  float* mr32;                  // n x F x 3: float |loop|, arg(loop), b(w)
+  float* mr32;                  // n x F x 4: float |loop|, arg(loop), b(w), error bound
   int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite
   int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
   int32_t* rcount;              // 1
+  int32_t* flist;               // n: candidates whose whole sweep is recomputed in FP64
+  int32_t* fcount;              // 1
   // mixed fit: partial right-hand sides sum_{t in part} (y_t - yss) z_e^(t - t_part), complex,
   // layout [part][mode][list slot]
   double2* atbp;
@@ -82,6 +91,7 @@ enum BatchKernel : int {
   BK_FITCF,         // key NB
   BK_SWEEP_MARK32,  // key NS
   BK_REFINE64,      // key NS
+  BK_FULL64,        // key NS
   BK_SCAN32,
   BK_COUNT
 };

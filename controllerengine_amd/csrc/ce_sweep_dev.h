@@ -162,145 +162,146 @@ __device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int
   b = num / (M::sqrt(R(1) + Ps * Ps) * M::sqrt(R(1) + Cs * Cs));
 }
 
-// Float sweep on the Hessenberg form (mixed chain only; the FP64 chain keeps the elimination
-// above for parity).  A1 = Q H Q^T by Householder reflectors, so
-//   G(z) = K^T (zI - A1)^-1 L = (Q^T K)^T (zI - H)^-1 (Q^T L),
-// and each frequency costs an O(NS^2) solve of the Hessenberg system (partial pivoting between
-// the two candidate rows of each column) instead of an O(NS^3) dense elimination.  The pivots are
-// inverted with the hardware reciprocal (1 ulp): these float values only locate the points that
-// are recomputed in FP64, with guard bands far wider than the rounding.
+// Sweep of the mixed chain on the Hessenberg form (the FP64 chain keeps the dense elimination
+// above for parity), in delta form: A1 - I = Q H Q^T by Householder reflectors, so
+//   G(z) = K^T (zI - A1)^-1 L = (Q^T K)^T ((z - 1) I - H)^-1 (Q^T L),
+// and each frequency costs an O(NS^2) solve of the Hessenberg system instead of an O(NS^3) dense
+// elimination.  The solve is FP64: the sampled plants put the eigenvalues of A1 within O(ts) of
+// 1, where (z - 1) I - H at the low frequencies is conditioned far beyond what float carries
+// (relative errors of 1e-2 in a float solve).  The loop value is then rounded to float and the
+// tail (|loop|, arg, b) evaluated in float; these values only locate the points that are
+// recomputed with the dense FP64 elimination.
 template <int NS>
 __device__ __forceinline__ void hess_prep(const double* __restrict__ oa, const float* __restrict__ gg,
-                                          float (&H)[NS][NS], float (&kq)[NS], float (&lq)[NS]) {
+                                          double (&H)[NS][NS], double (&kq)[NS], double (&lq)[NS], double& hnorm) {
+  // delta form: A1 - I (its diagonal is 1 - O(ts) for a sampled plant)
 #pragma unroll
   for (int i = 0; i < NS; i++) {
 #pragma unroll
-    for (int j = 0; j < NS; j++) H[i][j] = (float)oa[i * NS + j];
-    kq[i] = gg[i];
-    lq[i] = gg[NS + i];
+    for (int j = 0; j < NS; j++) H[i][j] = oa[i * NS + j] - (i == j ? 1.0 : 0.0);
+    kq[i] = (double)gg[i];
+    lq[i] = (double)gg[NS + i];
   }
 #pragma unroll
   for (int c = 0; c + 2 < NS; c++) {
-    float sig = 0.0f;
+    double sig = 0.0;
 #pragma unroll
-    for (int i = c + 2; i < NS; i++) sig = __builtin_fmaf(H[i][c], H[i][c], sig);
-    if (sig == 0.0f) continue;
-    const float alpha = H[c + 1][c];
-    const float nrm = __builtin_sqrtf(__builtin_fmaf(alpha, alpha, sig));
-    const float beta = alpha >= 0.0f ? -nrm : nrm;
-    float v[NS];
+    for (int i = c + 2; i < NS; i++) sig = __builtin_fma(H[i][c], H[i][c], sig);
+    if (sig == 0.0) continue;
+    const double alpha = H[c + 1][c];
+    const double nrm = __builtin_sqrt(__builtin_fma(alpha, alpha, sig));
+    const double beta = alpha >= 0.0 ? -nrm : nrm;
+    double v[NS];
 #pragma unroll
-    for (int i = 0; i < NS; i++) v[i] = (i == c + 1) ? alpha - beta : (i > c + 1 ? H[i][c] : 0.0f);
-    const float vtv = __builtin_fmaf(v[c + 1], v[c + 1], sig);
-    const float tau = 2.0f / vtv;
+    for (int i = 0; i < NS; i++) v[i] = (i == c + 1) ? alpha - beta : (i > c + 1 ? H[i][c] : 0.0);
+    const double vtv = __builtin_fma(v[c + 1], v[c + 1], sig);
+    const double tau = 2.0 / vtv;
     // H <- P H, then H <- H P, with P = I - tau v v^T (v zero above row c+1)
 #pragma unroll
     for (int j = 0; j < NS; j++) {
-      float sdot = 0.0f;
+      double sdot = 0.0;
 #pragma unroll
-      for (int i = c + 1; i < NS; i++) sdot = __builtin_fmaf(v[i], H[i][j], sdot);
-      const float f = tau * sdot;
+      for (int i = c + 1; i < NS; i++) sdot = __builtin_fma(v[i], H[i][j], sdot);
+      const double f = tau * sdot;
 #pragma unroll
-      for (int i = c + 1; i < NS; i++) H[i][j] = __builtin_fmaf(-f, v[i], H[i][j]);
+      for (int i = c + 1; i < NS; i++) H[i][j] = __builtin_fma(-f, v[i], H[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < NS; i++) {
-      float sdot = 0.0f;
+      double sdot = 0.0;
 #pragma unroll
-      for (int j = c + 1; j < NS; j++) sdot = __builtin_fmaf(H[i][j], v[j], sdot);
-      const float f = tau * sdot;
+      for (int j = c + 1; j < NS; j++) sdot = __builtin_fma(H[i][j], v[j], sdot);
+      const double f = tau * sdot;
 #pragma unroll
-      for (int j = c + 1; j < NS; j++) H[i][j] = __builtin_fmaf(-f, v[j], H[i][j]);
+      for (int j = c + 1; j < NS; j++) H[i][j] = __builtin_fma(-f, v[j], H[i][j]);
     }
-    float sk = 0.0f, sl = 0.0f;
+    double sk = 0.0, sl = 0.0;
 #pragma unroll
     for (int i = c + 1; i < NS; i++) {
-      sk = __builtin_fmaf(v[i], kq[i], sk);
-      sl = __builtin_fmaf(v[i], lq[i], sl);
+      sk = __builtin_fma(v[i], kq[i], sk);
+      sl = __builtin_fma(v[i], lq[i], sl);
     }
-    const float fk = tau * sk, fl = tau * sl;
+    const double fk = tau * sk, fl = tau * sl;
 #pragma unroll
     for (int i = c + 1; i < NS; i++) {
-      kq[i] = __builtin_fmaf(-fk, v[i], kq[i]);
-      lq[i] = __builtin_fmaf(-fl, v[i], lq[i]);
+      kq[i] = __builtin_fma(-fk, v[i], kq[i]);
+      lq[i] = __builtin_fma(-fl, v[i], lq[i]);
     }
   }
+  // ||H||_F, for the error estimate of the point solves
+  double hf = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++)
+#pragma unroll
+    for (int j = 0; j < NS; j++) hf = __builtin_fma(H[i][j], H[i][j], hf);
+  hnorm = __builtin_sqrt(hf);
 }
 
+// One frequency: G = kq^T M^-1 lq with M = (z - 1) I - H upper Hessenberg, by Gaussian elimination
+// with partial pivoting between the two candidate rows of each column, M = P L U.  With
+// w = L^-1 P^T lq (the eliminated right-hand side) and y = U^-T kq (forward substitution, run
+// row by row as the rows of U are produced), G = kq^T U^-1 w = y^T w, so no row of U is kept:
+// the live state is the carried row, the partly reduced kq and the running sum.
+// err: relative error of the float loop value (and, in radians, of its phase): the FP64 estimate
+//   SWEEP_ERR_SAFETY eps_d (||M|| / min |pivot|) ||y|| ||w|| / |G|
+// plus SWEEP_ERR_FLOOR for the float rounding of G and the float tail; 4 when unknown.
 template <int NS>
-__device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, const float (&H)[NS][NS],
-                                           const float (&kq)[NS], const float (&lq)[NS], int w, float& m, float& raw,
-                                           float& b) {
-  const float zr = (float)tab<double>(P, P->off_zr)[w], zi = (float)tab<double>(P, P->off_zi)[w];
-  // M = zI - H, upper Hessenberg; only entries with j >= i - 1 are used
-  float Mr[NS][NS], Mi[NS][NS], vr[NS], vi[NS];
+__device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, const double (&H)[NS][NS],
+                                           const double (&kq)[NS], const double (&lq)[NS], double hnorm,
+                                           int w, float& m, float& raw, float& b, float& err) {
+  const double zmr = tab<double>(P, P->off_zm1)[w], zi = tab<double>(P, P->off_zi)[w];
+  // carried row (row k of the elimination before its pivot choice), its right-hand side, and
+  // kq reduced by the rows of U produced so far
+  double cr[NS], ci[NS], yr[NS], yi[NS];
 #pragma unroll
-  for (int i = 0; i < NS; i++) {
-#pragma unroll
-    for (int j = 0; j < NS; j++) {
-      Mr[i][j] = (i == j ? zr : 0.0f) - H[i][j];
-      Mi[i][j] = (i == j ? zi : 0.0f);
-    }
-    vr[i] = lq[i];
-    vi[i] = 0.0f;
+  for (int j = 0; j < NS; j++) {
+    cr[j] = (j == 0 ? zmr : 0.0) - H[0][j];
+    ci[j] = (j == 0 ? zi : 0.0);
+    yr[j] = kq[j];
+    yi[j] = 0.0;
   }
+  double vr = lq[0], vi = 0.0;
+  double sr = 0.0, si = 0.0, y2 = 0.0, w2 = 0.0, pmin2 = __builtin_inf();
 #pragma unroll
-  for (int k = 0; k + 1 < NS; k++) {
-    const float a = Mr[k][k] * Mr[k][k] + Mi[k][k] * Mi[k][k];
-    const float c = Mr[k + 1][k] * Mr[k + 1][k] + Mi[k + 1][k] * Mi[k + 1][k];
-    const bool sw = c > a;
-#pragma unroll
-    for (int j = k; j < NS; j++) {
-      const float ar = Mr[k][j], ai = Mi[k][j], br = Mr[k + 1][j], bi = Mi[k + 1][j];
-      Mr[k][j] = sw ? br : ar;
-      Mi[k][j] = sw ? bi : ai;
-      Mr[k + 1][j] = sw ? ar : br;
-      Mi[k + 1][j] = sw ? ai : bi;
-    }
-    {
-      const float ar = vr[k], ai = vi[k], br = vr[k + 1], bi = vi[k + 1];
-      vr[k] = sw ? br : ar;
-      vi[k] = sw ? bi : ai;
-      vr[k + 1] = sw ? ar : br;
-      vi[k + 1] = sw ? ai : bi;
-    }
-    const float pr = Mr[k][k], pi = Mi[k][k];
-    const float inv = __builtin_amdgcn_rcpf(pr * pr + pi * pi);
-    const float lr = (Mr[k + 1][k] * pr + Mi[k + 1][k] * pi) * inv;
-    const float li = (Mi[k + 1][k] * pr - Mr[k + 1][k] * pi) * inv;
+  for (int k = 0; k < NS; k++) {
+    // row k + 1 of M: -H[k+1][j], plus z - 1 on the diagonal; its column-k entry is real
+    const double nk = (k + 1 < NS) ? -H[k + 1 < NS ? k + 1 : k][k] : 0.0;
+    const bool sw = (k + 1 < NS) && nk * nk > cr[k] * cr[k] + ci[k] * ci[k];
+    const double ukr = sw ? nk : cr[k], uki = sw ? 0.0 : ci[k];
+    const double okr = sw ? cr[k] : nk, oki = sw ? ci[k] : 0.0;
+    const double nv = (k + 1 < NS) ? lq[k + 1 < NS ? k + 1 : k] : 0.0;
+    const double wr = sw ? nv : vr, wi = sw ? 0.0 : vi;
+    const double ovr = sw ? vr : nv, ovi = sw ? vi : 0.0;
+    const double p2 = ukr * ukr + uki * uki;
+    pmin2 = __builtin_fmin(pmin2, p2);
+    const double inv = 1.0 / p2;
+    // l = o_k / u_k, y_k = kq'_k / u_k
+    const double lr = (okr * ukr + oki * uki) * inv, li = (oki * ukr - okr * uki) * inv;
+    const double ykr = (yr[k] * ukr + yi[k] * uki) * inv, yki = (yi[k] * ukr - yr[k] * uki) * inv;
 #pragma unroll
     for (int j = k + 1; j < NS; j++) {
-      Mr[k + 1][j] = Mr[k + 1][j] - (lr * Mr[k][j] - li * Mi[k][j]);
-      Mi[k + 1][j] = Mi[k + 1][j] - (lr * Mi[k][j] + li * Mr[k][j]);
+      const double njr = (j == k + 1 ? zmr : 0.0) - H[k + 1 < NS ? k + 1 : k][j], nji = (j == k + 1 ? zi : 0.0);
+      const double ujr = sw ? njr : cr[j], uji = sw ? nji : ci[j];
+      const double ojr = sw ? cr[j] : njr, oji = sw ? ci[j] : nji;
+      cr[j] = ojr - (lr * ujr - li * uji);
+      ci[j] = oji - (lr * uji + li * ujr);
+      yr[j] = yr[j] - (ujr * ykr - uji * yki);
+      yi[j] = yi[j] - (ujr * yki + uji * ykr);
     }
-    const float qr = lr * vr[k] - li * vi[k], qi = lr * vi[k] + li * vr[k];
-    vr[k + 1] = vr[k + 1] - qr;
-    vi[k + 1] = vi[k + 1] - qi;
+    vr = ovr - (lr * wr - li * wi);
+    vi = ovi - (lr * wi + li * wr);
+    sr = sr + (ykr * wr - yki * wi);
+    si = si + (ykr * wi + yki * wr);
+    y2 = y2 + (ykr * ykr + yki * yki);
+    w2 = w2 + (wr * wr + wi * wi);
   }
-  float xr[NS], xi[NS];
-  bool fin = true;
-#pragma unroll
-  for (int i = NS - 1; i >= 0; i--) {
-    float sr = vr[i], si = vi[i];
-#pragma unroll
-    for (int j = i + 1; j < NS; j++) {
-      sr = sr - (Mr[i][j] * xr[j] - Mi[i][j] * xi[j]);
-      si = si - (Mr[i][j] * xi[j] + Mi[i][j] * xr[j]);
-    }
-    const float pr = Mr[i][i], pi = Mi[i][i];
-    const float inv = __builtin_amdgcn_rcpf(pr * pr + pi * pi);
-    xr[i] = (sr * pr + si * pi) * inv;
-    xi[i] = (si * pr - sr * pi) * inv;
-    fin = fin && __builtin_isfinite(xr[i]) && __builtin_isfinite(xi[i]);
-  }
-  if (!fin) return false;
-  float gr = 0.0f, gi = 0.0f;
-#pragma unroll
-  for (int i = 0; i < NS; i++) {
-    gr = __builtin_fmaf(kq[i], xr[i], gr);
-    gi = __builtin_fmaf(kq[i], xi[i], gi);
-  }
-  sweep_tail<float>(P, w, gr, gi, m, raw, b);
+  const double g2 = sr * sr + si * si;
+  if (!(pmin2 > 0.0) || !__builtin_isfinite(g2)) return false;
+  // ||M|| <= sqrt(NS) |z - 1| + ||H||_F
+  const double mn = __builtin_sqrt((double)NS) * __builtin_sqrt(zmr * zmr + zi * zi) + hnorm;
+  const double e = (double)SWEEP_ERR_SAFETY * 1.1102230246251565e-16 * mn * __builtin_sqrt((y2 * w2) / (pmin2 * g2));
+  err = (e <= 1.0) ? (float)e + SWEEP_ERR_FLOOR : 4.0f;  // NaN and overflow: 4 (every guard then holds)
+  sweep_tail<float>(P, w, (float)sr, (float)si, m, raw, b);
   return true;
 }
 

@@ -1,0 +1,48 @@
+"""The chain at the full benchmark sizes (c2: 65,536 candidates, c3: 262,144), checked through
+properties that do not need the oracle to run the whole block (GPU):
+
+* FP64 rows of a seeded sample of candidates equal the oracle's rows for those candidates run
+  alone (each candidate's rows depend only on its own inputs);
+* the mixed chain matches the FP64 chain on every candidate of the block: |dGM| <= 0.01 dB,
+  |dPM| <= 0.01 deg, |db| <= 1e-4, identical pass/fail and stability flags;
+* two mixed evaluations of the block are bit-identical (the second stream, the gated simulation
+  and the atomics of the compaction and refinement list do not make the rows depend on timing).
+"""
+import numpy as np
+import pytest
+
+from controllerengine_amd import PRECISION_FP64, PRECISION_MIXED, metrics, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def same_rows(a, b):
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+@pytest.mark.parametrize("name,sample", [("c2", 512), ("c3", 256)])
+def test_full_block(engine, name, sample):
+    p = workloads.CONFIGS[name]()
+    n = p.gains.shape[0]
+    print("case", name, "full block n=%d" % n, flush=True)
+    _, r64 = engine.evaluate(p, PRECISION_FP64)
+    _, m1 = engine.evaluate(p, PRECISION_MIXED)
+    _, m2 = engine.evaluate(p, PRECISION_MIXED)
+
+    idx = np.sort(np.random.default_rng(11).choice(n, sample, replace=False))
+    _, o64, _ = oracle.evaluate(p, gains=p.gains[idx], info=p.info[idx])
+    ok = same_rows(r64[idx], o64)
+    assert ok.all(), "FP64 rows of the full block differ from the oracle in %d cells" % int((~ok).sum())
+
+    rep = metrics.compare(p, r64, m1)
+    print(rep, flush=True)
+    assert rep["n_margin_rows"] > 0
+    assert rep["max_dgm_db"] <= 0.01
+    assert rep["max_dpm_deg"] <= 0.01
+    assert rep["max_db"] <= 1e-4
+    assert rep["passfail_mismatch"] == 0
+    assert rep["stable_mismatch"] == 0
+
+    ok = same_rows(m1, m2)
+    assert ok.all(), "two mixed evaluations differ in %d cells" % int((~ok).sum())
