@@ -78,6 +78,7 @@ struct ce_engine {
   hipStream_t last_stream = nullptr;
   JitCache* jit = nullptr;
   int last_sim_jit = 0;
+  int last_mixed = 0;
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
   // simulation and the fit (forked after the compaction, joined before the crossing scan)
   hipStream_t side = nullptr;
@@ -189,6 +190,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   const bool mixed = precision == CE_PRECISION_MIXED;
   if (precision != CE_PRECISION_FP64 && !mixed)
     throw std::runtime_error("ce_evaluate: unknown precision " + std::to_string(precision));
+  E->last_mixed = mixed ? 1 : 0;
   B.mag_raw = (double*)ensure(E->mag_raw, nn * (size_t)F * (mixed ? 3 : 2) * sizeof(double));
   B.bmin = (unsigned long long*)ensure(E->bmin, nn * sizeof(unsigned long long));
   B.sweep_fail = (int32_t*)ensure(E->sfail, nn * sizeof(int32_t));
@@ -383,6 +385,24 @@ extern "C" CE_EXPORT int ce_engine_stage_times(ce_engine* E, double* ms, int n_m
 
 // 1 when the last evaluation's simulation stage ran the runtime-specialised kernel, else 0.
 extern "C" CE_EXPORT int ce_engine_last_sim_jit(ce_engine* E) { return E ? E->last_sim_jit : 0; }
+
+// Mixed sweep counts of the last evaluation: points recomputed in FP64 and candidates whose whole
+// sweep was.  Waits for the device.
+extern "C" CE_EXPORT int ce_engine_sweep_counts(ce_engine* E, int* refined_points, int* fp64_candidates) {
+  if (!E || !E->last_mixed || !E->rcount.p || !E->fcount.p) return 1;
+  try {
+    HIPCHK(hipSetDevice(E->device));
+    HIPCHK(hipDeviceSynchronize());
+    int32_t r = 0, f = 0;
+    HIPCHK(hipMemcpy(&r, E->rcount.p, sizeof(r), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&f, E->fcount.p, sizeof(f), hipMemcpyDeviceToHost));
+    if (refined_points) *refined_points = r;
+    if (fp64_candidates) *fp64_candidates = f;
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
 
 extern "C" CE_EXPORT int ce_detmath_device(int fn, const double* x, const double* y, double* out, int n, void* stream) {
   ce::launch_detmath(fn, x, y, out, n, (hipStream_t)stream);

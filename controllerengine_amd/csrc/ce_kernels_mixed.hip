@@ -359,19 +359,55 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   bool failp = false;
   float* mr = B.mr32 + (size_t)c * F * 4;
   const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
+  // The Hessenberg form is the same for every lane: it is reduced in registers, then kept in LDS
+  // (H, Q^T K, Q^T L) and read by broadcast, which leaves the registers to the point solves.
+  __shared__ double sh[NS * NS + 2 * NS];
+  double hnorm;
+  {
+    double H[NS][NS], kq[NS], lq[NS];
+    hess_prep<NS>(oa, gg, H, kq, lq, hnorm);
+    __syncthreads();  // a batched launch reuses sh for its next candidate
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) sh[i * NS + j] = H[i][j];
+        sh[NS * NS + i] = kq[i];
+        sh[NS * NS + NS + i] = lq[i];
+      }
+    }
+    __syncthreads();
+  }
+  const double* zm1t = tab<double>(P, P->off_zm1);
+  const double* zit = tab<double>(P, P->off_zi);
+  const double* prt = tab<double>(P, P->off_pr);
+  const double* pit = tab<double>(P, P->off_pi);
+  const double* pat = tab<double>(P, P->off_pabs);
+  const float alpha = (float)P->alpha;
   // the sweep points go to memory and are read back below, so that nothing else is live across
-  // the register-heavy elimination
-  double H[NS][NS], kq[NS], lq[NS], hnorm;
-  hess_prep<NS>(oa, gg, H, kq, lq, hnorm);
+  // the register-heavy elimination; each point's table values are loaded one point ahead
+  double nzr = 0.0, nzi = 0.0, npr = 0.0, npi = 0.0, npa = 0.0;
+  int npok = 0;
+  if (nk > 0) {
+    nzr = zm1t[w0]; nzi = zit[w0]; npr = prt[w0]; npi = pit[w0]; npa = pat[w0]; npok = pok[w0];
+  }
 #pragma unroll 1
   for (int k = 0; k < nk; k++) {
     const int w = w0 + k;
+    // keeps the LDS reads of the Hessenberg data inside the loop (not hoisted into registers)
+    __asm__ volatile("" ::: "memory");
+    const double zr = nzr, zi = nzi;
+    const float pr = (float)npr, pim = (float)npi, pa = (float)npa;
+    const int ok = npok;
+    if (k + 1 < nk) {
+      nzr = zm1t[w + 1]; nzi = zit[w + 1]; npr = prt[w + 1]; npi = pit[w + 1]; npa = pat[w + 1]; npok = pok[w + 1];
+    }
     float mm = __builtin_nanf(""), rr = __builtin_nanf(""), b2 = __builtin_nanf(""), e2 = 1.0f;
-    if (!pok[w]) {
+    if (!ok) {
       failp = true;
     } else {
       float m1, r1, bq, e1;
-      if (hess_point<NS>(P, H, kq, lq, hnorm, w, m1, r1, bq, e1)) { mm = m1; rr = r1; b2 = bq; e2 = e1; }
+      if (hess_point<NS>(sh, hnorm, zr, zi, pr, pim, pa, alpha, m1, r1, bq, e1)) { mm = m1; rr = r1; b2 = bq; e2 = e1; }
     }
     mr[4 * w] = mm;
     mr[4 * w + 1] = rr;

@@ -147,19 +147,23 @@ __device__ __forceinline__ bool sweep_point(const ProblemDev* __restrict__ P, co
   return true;
 }
 
-// |loop|, arg(loop) and b(w) from G(z_w) (reference src/allmargin.cu:104-117).
+// |loop|, arg(loop) and b(w) from G(z_w) and the plant response P(z_w) = pr + i pim, |P| = pabs
+// (reference src/allmargin.cu:104-117).
 template <typename R>
-__device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int w, R gr, R gi, R& m, R& raw, R& b) {
+__device__ __forceinline__ void sweep_tail_vals(R pr, R pim, R pabs, R alpha, R gr, R gi, R& m, R& raw, R& b) {
   using M = SweepMath<R>;
-  const R pr = (R)tab<double>(P, P->off_pr)[w], pim = (R)tab<double>(P, P->off_pi)[w];
   const R rr = pr * gr - pim * gi, ri = pr * gi + pim * gr;
   m = M::hyp(rr, ri);
   raw = M::arg(ri, rr);
-  const R alpha = (R)P->alpha;
-  const R Ps = alpha * (R)tab<double>(P, P->off_pabs)[w];
+  const R Ps = alpha * pabs;
   const R Cs = M::hyp(gr, gi) / alpha;
   const R num = M::hyp(R(1) + rr, ri);
   b = num / (M::sqrt(R(1) + Ps * Ps) * M::sqrt(R(1) + Cs * Cs));
+}
+template <typename R>
+__device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int w, R gr, R gi, R& m, R& raw, R& b) {
+  sweep_tail_vals<R>((R)tab<double>(P, P->off_pr)[w], (R)tab<double>(P, P->off_pi)[w],
+                     (R)tab<double>(P, P->off_pabs)[w], (R)P->alpha, gr, gi, m, raw, b);
 }
 
 // Sweep of the mixed chain on the Hessenberg form (the FP64 chain keeps the dense elimination
@@ -245,17 +249,22 @@ __device__ __forceinline__ void hess_prep(const double* __restrict__ oa, const f
 // err: relative error of the float loop value (and, in radians, of its phase): the FP64 estimate
 //   SWEEP_ERR_SAFETY eps_d (||M|| / min |pivot|) ||y|| ||w|| / |G|
 // plus SWEEP_ERR_FLOOR for the float rounding of G and the float tail; 4 when unknown.
+// The Hessenberg data come from a wave-uniform LDS block sh = [H (row-major), Q^T K, Q^T L];
+// (zmr, zi) = z - 1, and (pr, pim, pabs) the plant response at the point.
 template <int NS>
-__device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, const double (&H)[NS][NS],
-                                           const double (&kq)[NS], const double (&lq)[NS], double hnorm,
-                                           int w, float& m, float& raw, float& b, float& err) {
-  const double zmr = tab<double>(P, P->off_zm1)[w], zi = tab<double>(P, P->off_zi)[w];
+__device__ __forceinline__ bool hess_point(const double* __restrict__ sh, double hnorm, double zmr, double zi,
+                                           float pr, float pim, float pabs, float alpha, float& m, float& raw,
+                                           float& b, float& err) {
+  const double* H0 = sh;
+  const double* kq = sh + NS * NS;
+  const double* lq = sh + NS * NS + NS;
+#define H(i, j) H0[(i) * NS + (j)]
   // carried row (row k of the elimination before its pivot choice), its right-hand side, and
   // kq reduced by the rows of U produced so far
   double cr[NS], ci[NS], yr[NS], yi[NS];
 #pragma unroll
   for (int j = 0; j < NS; j++) {
-    cr[j] = (j == 0 ? zmr : 0.0) - H[0][j];
+    cr[j] = (j == 0 ? zmr : 0.0) - H(0, j);
     ci[j] = (j == 0 ? zi : 0.0);
     yr[j] = kq[j];
     yi[j] = 0.0;
@@ -265,7 +274,7 @@ __device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, con
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     // row k + 1 of M: -H[k+1][j], plus z - 1 on the diagonal; its column-k entry is real
-    const double nk = (k + 1 < NS) ? -H[k + 1 < NS ? k + 1 : k][k] : 0.0;
+    const double nk = (k + 1 < NS) ? -H(k + 1 < NS ? k + 1 : k, k) : 0.0;
     const bool sw = (k + 1 < NS) && nk * nk > cr[k] * cr[k] + ci[k] * ci[k];
     const double ukr = sw ? nk : cr[k], uki = sw ? 0.0 : ci[k];
     const double okr = sw ? cr[k] : nk, oki = sw ? ci[k] : 0.0;
@@ -280,7 +289,7 @@ __device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, con
     const double ykr = (yr[k] * ukr + yi[k] * uki) * inv, yki = (yi[k] * ukr - yr[k] * uki) * inv;
 #pragma unroll
     for (int j = k + 1; j < NS; j++) {
-      const double njr = (j == k + 1 ? zmr : 0.0) - H[k + 1 < NS ? k + 1 : k][j], nji = (j == k + 1 ? zi : 0.0);
+      const double njr = (j == k + 1 ? zmr : 0.0) - H(k + 1 < NS ? k + 1 : k, j), nji = (j == k + 1 ? zi : 0.0);
       const double ujr = sw ? njr : cr[j], uji = sw ? nji : ci[j];
       const double ojr = sw ? cr[j] : njr, oji = sw ? ci[j] : nji;
       cr[j] = ojr - (lr * ujr - li * uji);
@@ -301,8 +310,9 @@ __device__ __forceinline__ bool hess_point(const ProblemDev* __restrict__ P, con
   const double mn = __builtin_sqrt((double)NS) * __builtin_sqrt(zmr * zmr + zi * zi) + hnorm;
   const double e = (double)SWEEP_ERR_SAFETY * 1.1102230246251565e-16 * mn * __builtin_sqrt((y2 * w2) / (pmin2 * g2));
   err = (e <= 1.0) ? (float)e + SWEEP_ERR_FLOOR : 4.0f;  // NaN and overflow: 4 (every guard then holds)
-  sweep_tail<float>(P, w, (float)sr, (float)si, m, raw, b);
+  sweep_tail_vals<float>(pr, pim, pabs, alpha, (float)sr, (float)si, m, raw, b);
   return true;
+#undef H
 }
 
 }  // namespace ce

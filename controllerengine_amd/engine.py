@@ -89,6 +89,13 @@ class Engine:
         k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)
         return [buf[i] for i in range(k)]
 
+    def sweep_counts(self):
+        """(FP64-refined points, candidates swept entirely in FP64) of the last mixed evaluate."""
+        r, f = ctypes.c_int(0), ctypes.c_int(0)
+        if _abi.lib().ce_engine_sweep_counts(self.h, ctypes.byref(r), ctypes.byref(f)) != 0:
+            raise RuntimeError("ce_engine_sweep_counts: the last evaluation was not mixed")
+        return r.value, f.value
+
     def set_profiling(self, on=True):
         _abi.lib().ce_engine_set_profiling(self.h, 1 if on else 0)
 

@@ -148,6 +148,10 @@ CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
  * launch order.  Off by default (the events add small gaps between kernels). */
 CE_EXPORT int ce_engine_set_profiling(ce_engine* eng, int on);
 CE_EXPORT int ce_engine_kernel_times(ce_engine* eng, char* names, int names_len, double* ms, int n_max);
+/* Mixed-sweep counts of the last ce_evaluate in mixed precision (no counterpart in the reference):
+ * frequency points recomputed with the FP64 elimination, and candidates whose whole sweep was.
+ * Waits for the device.  Returns 1 when the last call was not mixed. */
+CE_EXPORT int ce_engine_sweep_counts(ce_engine* eng, int* refined_points, int* fp64_candidates);
 
 /* ---------------------------------------------------------------------------------------------
  * Batched evaluation (no counterpart in the reference, which evaluates one plant per call; its

@@ -29,6 +29,9 @@ def test_full_block(engine, name, sample):
     _, r64 = engine.evaluate(p, PRECISION_FP64)
     _, m1 = engine.evaluate(p, PRECISION_MIXED)
     _, m2 = engine.evaluate(p, PRECISION_MIXED)
+    refined, full = engine.sweep_counts()
+    print("mixed sweep: %d points refined in FP64, %d candidates swept in FP64" % (refined, full), flush=True)
+    assert 0 <= full <= n and 0 <= refined <= 64 * n
 
     idx = np.sort(np.random.default_rng(11).choice(n, sample, replace=False))
     _, o64, _ = oracle.evaluate(p, gains=p.gains[idx], info=p.info[idx])
