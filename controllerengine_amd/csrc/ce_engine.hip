@@ -22,6 +22,8 @@ void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t
 void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_scan32(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_scan32_split(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_margin_combine(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
@@ -62,7 +64,7 @@ struct ce_engine {
     size_t cap = 0;
   };
   Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
-  Buf mr32, refined, rlist, rcount, flist, fcount, atbp;
+  Buf mr32, refined, rlist, rcount, flist, fcount, mres, atbp;
   // per-kernel timing (ce_engine_set_profiling)
   int profiling = 0;
   std::vector<hipEvent_t> mark_ev;
@@ -203,6 +205,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     B.rcount = (int32_t*)ensure(E->rcount, sizeof(int32_t));
     B.flist = (int32_t*)ensure(E->flist, nn * sizeof(int32_t));
     B.fcount = (int32_t*)ensure(E->fcount, sizeof(int32_t));
+    B.mres = (double*)ensure(E->mres, nn * 6 * sizeof(double));
     // time parts of the right-hand-side pass: enough waves to fill the chip, >= 64 samples each
     const int mb = (nt + 7) / 8;
     const long long waves = (long long)mb * (long long)((nn + 63) / 64);
@@ -276,6 +279,9 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     }
     launch_eig_nom(H.ns, P, B, E->side);
     if (!launch_mark_refine(ns, F, P, B, E->side)) throw std::runtime_error("no sweep kernel for these dimensions");
+    // the crossing scan needs only the sweep: it runs here, beside the fit, and leaves its minima
+    // for the row updates after the fit
+    launch_scan32_split(P, B, E->side);
     HIPCHK(hipEventRecord(E->ev_join, E->side));
   }
   HIPCHK(hipEventRecord(E->ev[1], st));
@@ -301,7 +307,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // stage 4: margin sweep, crossings and total score
   if (fork) {
     HIPCHK(hipStreamWaitEvent(st, E->ev_join, 0));
-    launch_scan32(P, B, st);
+    launch_margin_combine(P, B, st);
   } else if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st))) {
     throw std::runtime_error("no sweep kernel for these dimensions");
   }
@@ -337,7 +343,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
                   &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
-                  &E->rcount, &E->flist, &E->fcount, &E->atbp, &E->gate})
+                  &E->rcount, &E->flist, &E->fcount, &E->mres, &E->atbp, &E->gate})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);
   if (E->blob_pinned) hipHostFree(E->blob_pinned);

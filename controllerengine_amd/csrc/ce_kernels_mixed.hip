@@ -643,6 +643,16 @@ __global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ 
   k_refine64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
+// The point refinement (first REFINE_BLOCKS workgroups) and the whole-candidate FP64 sweeps (the
+// rest) in one launch: they write disjoint candidates' points.
+constexpr int REFINE_BLOCKS = 1024, FULL64_BLOCKS = 256;
+template <int NS>
+__global__ __launch_bounds__(64) void k_refine_full64(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int bx = blockIdx.x;
+  if (bx < REFINE_BLOCKS) k_refine64_body<NS>(P, B, bx, REFINE_BLOCKS, 0, 0);
+  else k_full64_body<NS>(P, B, bx - REFINE_BLOCKS, FULL64_BLOCKS, 0, 0);
+}
+
 template <int NS>
 __global__ __launch_bounds__(64) void k_refine64_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
@@ -652,9 +662,27 @@ __global__ __launch_bounds__(64) void k_refine64_batch(BatchTab T) {
 }
 
 
+// Margin error and total score from the scan's minima (src/allmargin.cu:182-196).
+__device__ __forceinline__ void margin_score(const ProblemDev* __restrict__ P, double* sc, double* am, double GM,
+                                             double PM, double DM, double Bmin, bool crossed) {
+  const double EF = P->eig_fail_err;
+  const double err_gm = cedm::fmax_(0.0, P->target_gm - GM) / P->target_gm;
+  const double err_pm = crossed ? cedm::fmax_(0.0, P->target_pm - PM) / P->target_pm : 0.0;
+  const double err_b = cedm::fmax_(0.0, P->target_b - Bmin) / P->target_b;
+  const double margin = (err_gm * err_gm + err_pm * err_pm) + err_b * err_b;
+  sc[1] = sc[0];
+  sc[2] = margin;
+  am[0] = GM; am[1] = PM; am[2] = DM; am[3] = 1.0; am[4] = Bmin;
+  const double total = cedm::log((sc[0] + margin) + EPS30);
+  sc[0] = cedm::isfinite_(total) ? total : EF;
+}
+
 // Crossing scan (src/allmargin.cu:119-180) over the float values with the FP64 refinements in
 // place, one wave per candidate: each interval's contributions are independent once the phase is
 // unwrapped (chunk-wise prefix in double); the minima and flags reduce over the wave.
+// SPLIT: the scan runs beside the fit (second stream), so the behaviour score is not final yet: it
+// only stores its minima in B.mres, for k_margin_combine to apply after the fit.
+template <bool SPLIT>
 __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int c = bx;
   const int lane = threadIdx.x;
@@ -663,14 +691,15 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
   double* am = row + P->info_allmargin;
   const double UE = P->unstable_err, EF = P->eig_fail_err;
   const int flags = B.flags[c];
-  if (sc[0] >= UE) {
+  if (SPLIT && margin_skip_flags(B, c)) return;  // not swept; the combine step writes its row
+  if (!SPLIT && sc[0] >= UE) {
     if (lane == 0) {
       sc[1] = sc[0]; sc[2] = sc[0];
       am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
     }
     return;
   }
-  if (flags & F_NOMINAL_UNSTABLE) {
+  if (!SPLIT && (flags & F_NOMINAL_UNSTABLE)) {
     if (lane == 0) {
       sc[1] = sc[0];
       sc[2] = UE;
@@ -720,8 +749,12 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
   }
   if (fail || __ballot(failp)) {
     if (lane == 0) {
-      sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
-      am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+      if (SPLIT) {
+        B.mres[(size_t)c * 6 + 5] = 1.0;
+      } else {
+        sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
+        am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+      }
     }
     return;
   }
@@ -805,25 +838,57 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
   }
   crossed = __ballot(crossed) != 0ull;
   if (lane != 0) return;
-  const double err_gm = cedm::fmax_(0.0, P->target_gm - GM) / P->target_gm;
-  const double err_pm = crossed ? cedm::fmax_(0.0, P->target_pm - PM) / P->target_pm : 0.0;
-  const double err_b = cedm::fmax_(0.0, P->target_b - Bmin) / P->target_b;
-  const double margin = (err_gm * err_gm + err_pm * err_pm) + err_b * err_b;
-  sc[1] = sc[0];
-  sc[2] = margin;
-  am[0] = GM; am[1] = PM; am[2] = DM; am[3] = 1.0; am[4] = Bmin;
-  const double total = cedm::log((sc[0] + margin) + EPS30);
-  sc[0] = cedm::isfinite_(total) ? total : EF;
+  if (SPLIT) {
+    double* r = B.mres + (size_t)c * 6;
+    r[0] = GM; r[1] = PM; r[2] = DM; r[3] = Bmin; r[4] = crossed ? 1.0 : 0.0; r[5] = 0.0;
+  } else {
+    margin_score(P, sc, am, GM, PM, DM, Bmin, crossed);
+  }
+}
+
+// After the fit: the row updates of the crossing scan from its stored minima (lane per candidate);
+// the same branches, in the same order, as the unsplit scan.
+__global__ __launch_bounds__(256) void k_margin_combine(const ProblemDev* __restrict__ P, StageBufs B) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B.n) return;
+  double* row = B.info + (size_t)c * P->info_w;
+  double* sc = row + P->info_scoring;
+  double* am = row + P->info_allmargin;
+  const double UE = P->unstable_err, EF = P->eig_fail_err;
+  if (sc[0] >= UE) {
+    sc[1] = sc[0]; sc[2] = sc[0];
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    return;
+  }
+  if (B.flags[c] & F_NOMINAL_UNSTABLE) {
+    sc[1] = sc[0];
+    sc[2] = UE;
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;
+    const double total = cedm::log((sc[0] + UE) + EPS30);
+    sc[0] = cedm::isfinite_(total) ? total : EF;
+    return;
+  }
+  const double* r = B.mres + (size_t)c * 6;
+  if (r[5] != 0.0) {
+    sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
+    am[0] = 0.0; am[1] = 0.0; am[2] = 0.0; am[3] = 1.0; am[4] = 0.0;
+    return;
+  }
+  margin_score(P, sc, am, r[0], r[1], r[2], r[3], r[4] != 0.0);
 }
 
 __global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_scan32_body(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+  k_scan32_body<false>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+}
+
+__global__ __launch_bounds__(64) void k_scan32_split(const ProblemDev* __restrict__ P, StageBufs B) {
+  k_scan32_body<true>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 __global__ __launch_bounds__(64) void k_scan32_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
-    k_scan32_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+    k_scan32_body<false>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
 }
 
@@ -851,10 +916,8 @@ template <int NS>
 static void launch_mark_refine_t(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_sweep_mark32<NS>, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_sweep_mark32", st);
-  hipLaunchKernelGGL(k_refine64<NS>, dim3(1024), dim3(64), 0, st, P, B);
-  kmark("k_refine64", st);
-  hipLaunchKernelGGL(k_full64<NS>, dim3(256), dim3(64), 0, st, P, B);
-  kmark("k_full64", st);
+  hipLaunchKernelGGL(k_refine_full64<NS>, dim3(REFINE_BLOCKS + FULL64_BLOCKS), dim3(64), 0, st, P, B);
+  kmark("k_refine_full64", st);
 }
 
 // float sweep + marking + FP64 refinement (independent of the simulation and the fit)
@@ -893,6 +956,16 @@ void launch_gate_wait(const int* gate, int gen, hipStream_t st) {
 void launch_scan32(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   hipLaunchKernelGGL(k_scan32, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_scan32", st);
+}
+
+// the crossing scan beside the fit (after the refinement), then the row updates after the fit
+void launch_scan32_split(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan32_split, dim3(B.n), dim3(64), 0, st, P, B);
+  kmark("k_scan32", st);
+}
+void launch_margin_combine(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_margin_combine, dim3((B.n + 255) / 256), dim3(256), 0, st, P, B);
+  kmark("k_margin_combine", st);
 }
 
 bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {

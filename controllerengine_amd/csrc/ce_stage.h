@@ -47,6 +47,8 @@ struct StageBufs {
   int32_t* rcount;              // 1
   int32_t* flist;               // n: candidates whose whole sweep is recomputed in FP64
   int32_t* fcount;              // 1
+  double* mres;                 // n x 6: crossing-scan minima GM, PM, DM, b, crossed, failed (scan
+                                // beside the fit; NULL when the scan follows the fit)
   // mixed fit: partial right-hand sides sum_{t in part} (y_t - yss) z_e^(t - t_part), complex,
   // layout [part][mode][list slot]
   double2* atbp;
