@@ -592,7 +592,7 @@ void launch_eig_lin(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t
   const int key = eig_key(nt), cpw = 64 / eig_lanes(key);
   const dim3 grid((B.n + cpw - 1) / cpw);
   if (key == 4) hipLaunchKernelGGL((k_eig_lin<4, 1>), grid, dim3(64), 0, st, P, B);
-  else if (key == 8) hipLaunchKernelGGL((k_eig_lin<8, 1>), grid, dim3(64), 0, st, P, B);
+  else if (key == 8) hipLaunchKernelGGL((k_eig_lin<EIG8_G, 8 / EIG8_G>), grid, dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL((k_eig_lin<EIG16_G, 16 / EIG16_G>), grid, dim3(64), 0, st, P, B);
   kmark("k_eig_lin", st);
 }
@@ -602,7 +602,7 @@ void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t
   const int key = eig_key(m), cpw = 64 / eig_lanes(key);
   const dim3 grid((B.n + cpw - 1) / cpw);
   if (key == 4) hipLaunchKernelGGL((k_eig_nom<4, 1>), grid, dim3(64), 0, st, P, B);
-  else if (key == 8) hipLaunchKernelGGL((k_eig_nom<8, 1>), grid, dim3(64), 0, st, P, B);
+  else if (key == 8) hipLaunchKernelGGL((k_eig_nom<EIG8_G, 8 / EIG8_G>), grid, dim3(64), 0, st, P, B);
   else hipLaunchKernelGGL((k_eig_nom<EIG16_G, 16 / EIG16_G>), grid, dim3(64), 0, st, P, B);
   kmark("k_eig_nom", st);
 }
@@ -660,13 +660,13 @@ bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream
   switch (kind) {
     case BK_EIG_LIN:
       if (key == 4) hipLaunchKernelGGL((k_eig_lin_batch<4, 1>), g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL((k_eig_lin_batch<8, 1>), g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL((k_eig_lin_batch<EIG8_G, 8 / EIG8_G>), g, b, 0, st, T);
       else if (key == 16) hipLaunchKernelGGL((k_eig_lin_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
       else return false;
       return true;
     case BK_EIG_NOM:
       if (key == 4) hipLaunchKernelGGL((k_eig_nom_batch<4, 1>), g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL((k_eig_nom_batch<8, 1>), g, b, 0, st, T);
+      else if (key == 8) hipLaunchKernelGGL((k_eig_nom_batch<EIG8_G, 8 / EIG8_G>), g, b, 0, st, T);
       else if (key == 16) hipLaunchKernelGGL((k_eig_nom_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
       else return false;
       return true;

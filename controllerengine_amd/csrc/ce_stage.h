@@ -62,12 +62,16 @@ struct StageBufs {
 };
 
 // Lane layout of the eigen kernels.  Matrices of order <= 4, <= 8 and <= 16 (the kernel's key)
-// use 4, 8 and EIG16_G lanes per candidate; order-16 matrices put 16 / EIG16_G rows on a lane.
+// use 4, EIG8_G and EIG16_G lanes per candidate, with key / lanes rows on a lane.  At order 16,
 // 16 lanes (one row each) measured faster than 8 lanes with two rows each, whose doubled LDS per
-// wave halves the waves in flight (c3 LIN_A eigensolve 7.8 vs 8.0 ms).
+// wave halves the waves in flight (c3 LIN_A eigensolve 7.8 vs 8.0 ms).  At order 8, 4 lanes with
+// two rows each measured faster than 8 lanes (c4 order-8 LIN_A eigensolve 2.07 vs 2.55 ms; c2
+// unchanged): every lane of a group evaluates the shifts and Householder coefficients, so fewer
+// lanes per candidate do less redundant work.
 constexpr int EIG16_G = 16;
+constexpr int EIG8_G = 4;
 inline int eig_key(int order) { return order <= 4 ? 4 : (order <= 8 ? 8 : 16); }
-inline int eig_lanes(int key) { return key == 16 ? EIG16_G : key; }
+inline int eig_lanes(int key) { return key == 16 ? EIG16_G : (key == 8 ? EIG8_G : key); }
 
 // Batched launches: one launch of a stage kernel covers a group of problems (same kernel
 // template instance).  Problem j of the group owns the group-local blocks [start[j], start[j+1]);
