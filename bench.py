@@ -351,7 +351,10 @@ def main():
                           "c3": "random stable 8-state, 4 NL"}[args.config], n, cfg.time, cfg.freq, head),
                    "precision": head},
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
-        "roofline": {"bound": "valu", "kernel": dom, "launch_ms": dom_ms, "achieved": achieved, "peak": peak,
+        # compute-bound: the kernel issues vector FP32/FP64 instructions (no matrix op on this path);
+        # MI355X's published FP32 and FP64 peaks are the same for vector and matrix instructions
+        "roofline": {"bound": "mfma", "pipe": "valu", "kernel": dom, "launch_ms": dom_ms, "achieved": achieved,
+                     "peak": peak,
                      "unit": "TFLOP/s", "frac": (achieved / peak) if achieved is not None else None,
                      "traffic": traffic, "flops_per_launch": fl.get(dom)},
     }
