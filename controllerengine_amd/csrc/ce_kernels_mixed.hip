@@ -359,25 +359,12 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   bool failp = false;
   float* mr = B.mr32 + (size_t)c * F * 4;
   const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
-  // The Hessenberg form is the same for every lane: it is reduced in registers, then kept in LDS
-  // (H, Q^T K, Q^T L) and read by broadcast, which leaves the registers to the point solves.
+  // The Hessenberg form is the same for every lane: it is reduced and kept in LDS (H, Q^T K,
+  // Q^T L) and read by broadcast, which leaves the registers to the point solves.
   __shared__ double sh[NS * NS + 2 * NS];
   double hnorm;
-  {
-    double H[NS][NS], kq[NS], lq[NS];
-    hess_prep<NS>(oa, gg, H, kq, lq, hnorm);
-    __syncthreads();  // a batched launch reuses sh for its next candidate
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < NS; i++) {
-#pragma unroll
-        for (int j = 0; j < NS; j++) sh[i * NS + j] = H[i][j];
-        sh[NS * NS + i] = kq[i];
-        sh[NS * NS + NS + i] = lq[i];
-      }
-    }
-    __syncthreads();
-  }
+  __syncthreads();  // a batched launch reuses sh for its next candidate
+  hess_prep_lds<NS>(oa, gg, sh, hnorm);
   const double* zm1t = tab<double>(P, P->off_zm1);
   const double* zit = tab<double>(P, P->off_zi);
   const double* prt = tab<double>(P, P->off_pr);
@@ -573,12 +560,14 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_sweep_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
+// at least 3 waves per SIMD: the order-8 point solve otherwise holds 182 VGPRs (2 waves), with
+// its LDS reads of the Hessenberg data issued far ahead
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_mark32(const ProblemDev* __restrict__ P, StageBufs B) {
   k_sweep_mark32_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_sweep_mark32_batch(BatchTab T) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_mark32_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
     k_sweep_mark32_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);

@@ -241,6 +241,66 @@ __device__ __forceinline__ void hess_prep(const double* __restrict__ oa, const f
   hnorm = __builtin_sqrt(hf);
 }
 
+// The same reduction on the LDS block sh = [H (row-major), Q^T K, Q^T L] of one workgroup (one
+// wave): each reflector is formed by every lane from the same LDS values, then lane j updates
+// column j of H (H <- P H) and lanes NS, NS + 1 update Q^T K and Q^T L, then lane i updates row i
+// (H <- H P).  O(NS^2) per reflector per lane instead of O(NS^3), and no matrix in registers.
+template <int NS>
+__device__ __forceinline__ void hess_prep_lds(const double* __restrict__ oa, const float* __restrict__ gg,
+                                              double* sh, double& hnorm) {
+  const int lane = threadIdx.x;
+  double* H = sh;
+  double* kq = sh + NS * NS;
+  double* lq = sh + NS * NS + NS;
+  for (int e = lane; e < NS * NS; e += 64) H[e] = oa[e] - ((e / NS) == (e % NS) ? 1.0 : 0.0);
+  if (lane < NS) {
+    kq[lane] = (double)gg[lane];
+    lq[lane] = (double)gg[NS + lane];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c + 2 < NS; c++) {
+    double sig = 0.0;
+#pragma unroll
+    for (int i = c + 2; i < NS; i++) sig = __builtin_fma(H[i * NS + c], H[i * NS + c], sig);
+    if (sig == 0.0) continue;  // the same LDS values in every lane: uniform
+    const double alpha = H[(c + 1) * NS + c];
+    const double nrm = __builtin_sqrt(__builtin_fma(alpha, alpha, sig));
+    const double beta = alpha >= 0.0 ? -nrm : nrm;
+    double v[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) v[i] = (i == c + 1) ? alpha - beta : (i > c + 1 ? H[i * NS + c] : 0.0);
+    const double vtv = __builtin_fma(v[c + 1], v[c + 1], sig);
+    const double tau = 2.0 / vtv;
+    if (lane < NS + 2) {
+      // lane j < NS: column j of H; lane NS: Q^T K; lane NS + 1: Q^T L
+      double* col = lane < NS ? H + lane : (lane == NS ? kq : lq);
+      const int stride = lane < NS ? NS : 1;
+      double sdot = 0.0;
+#pragma unroll
+      for (int i = c + 1; i < NS; i++) sdot = __builtin_fma(v[i], col[i * stride], sdot);
+      const double f = tau * sdot;
+#pragma unroll
+      for (int i = c + 1; i < NS; i++) col[i * stride] = __builtin_fma(-f, v[i], col[i * stride]);
+    }
+    __syncthreads();
+    if (lane < NS) {
+      double* row = H + lane * NS;
+      double sdot = 0.0;
+#pragma unroll
+      for (int j = c + 1; j < NS; j++) sdot = __builtin_fma(row[j], v[j], sdot);
+      const double f = tau * sdot;
+#pragma unroll
+      for (int j = c + 1; j < NS; j++) row[j] = __builtin_fma(-f, v[j], row[j]);
+    }
+    __syncthreads();
+  }
+  double hf = 0.0;
+#pragma unroll
+  for (int e = 0; e < NS * NS; e++) hf = __builtin_fma(H[e], H[e], hf);
+  hnorm = __builtin_sqrt(hf);
+}
+
 // One frequency: G = kq^T M^-1 lq with M = (z - 1) I - H upper Hessenberg, by Gaussian elimination
 // with partial pivoting between the two candidate rows of each column, M = P L U.  With
 // w = L^-1 P^T lq (the eliminated right-hand side) and y = U^-T kq (forward substitution, run
