@@ -175,76 +175,11 @@ __device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int
 // (relative errors of 1e-2 in a float solve).  The loop value is then rounded to float and the
 // tail (|loop|, arg, b) evaluated in float; these values only locate the points that are
 // recomputed with the dense FP64 elimination.
-template <int NS>
-__device__ __forceinline__ void hess_prep(const double* __restrict__ oa, const float* __restrict__ gg,
-                                          double (&H)[NS][NS], double (&kq)[NS], double (&lq)[NS], double& hnorm) {
-  // delta form: A1 - I (its diagonal is 1 - O(ts) for a sampled plant)
-#pragma unroll
-  for (int i = 0; i < NS; i++) {
-#pragma unroll
-    for (int j = 0; j < NS; j++) H[i][j] = oa[i * NS + j] - (i == j ? 1.0 : 0.0);
-    kq[i] = (double)gg[i];
-    lq[i] = (double)gg[NS + i];
-  }
-#pragma unroll
-  for (int c = 0; c + 2 < NS; c++) {
-    double sig = 0.0;
-#pragma unroll
-    for (int i = c + 2; i < NS; i++) sig = __builtin_fma(H[i][c], H[i][c], sig);
-    if (sig == 0.0) continue;
-    const double alpha = H[c + 1][c];
-    const double nrm = __builtin_sqrt(__builtin_fma(alpha, alpha, sig));
-    const double beta = alpha >= 0.0 ? -nrm : nrm;
-    double v[NS];
-#pragma unroll
-    for (int i = 0; i < NS; i++) v[i] = (i == c + 1) ? alpha - beta : (i > c + 1 ? H[i][c] : 0.0);
-    const double vtv = __builtin_fma(v[c + 1], v[c + 1], sig);
-    const double tau = 2.0 / vtv;
-    // H <- P H, then H <- H P, with P = I - tau v v^T (v zero above row c+1)
-#pragma unroll
-    for (int j = 0; j < NS; j++) {
-      double sdot = 0.0;
-#pragma unroll
-      for (int i = c + 1; i < NS; i++) sdot = __builtin_fma(v[i], H[i][j], sdot);
-      const double f = tau * sdot;
-#pragma unroll
-      for (int i = c + 1; i < NS; i++) H[i][j] = __builtin_fma(-f, v[i], H[i][j]);
-    }
-#pragma unroll
-    for (int i = 0; i < NS; i++) {
-      double sdot = 0.0;
-#pragma unroll
-      for (int j = c + 1; j < NS; j++) sdot = __builtin_fma(H[i][j], v[j], sdot);
-      const double f = tau * sdot;
-#pragma unroll
-      for (int j = c + 1; j < NS; j++) H[i][j] = __builtin_fma(-f, v[j], H[i][j]);
-    }
-    double sk = 0.0, sl = 0.0;
-#pragma unroll
-    for (int i = c + 1; i < NS; i++) {
-      sk = __builtin_fma(v[i], kq[i], sk);
-      sl = __builtin_fma(v[i], lq[i], sl);
-    }
-    const double fk = tau * sk, fl = tau * sl;
-#pragma unroll
-    for (int i = c + 1; i < NS; i++) {
-      kq[i] = __builtin_fma(-fk, v[i], kq[i]);
-      lq[i] = __builtin_fma(-fl, v[i], lq[i]);
-    }
-  }
-  // ||H||_F, for the error estimate of the point solves
-  double hf = 0.0;
-#pragma unroll
-  for (int i = 0; i < NS; i++)
-#pragma unroll
-    for (int j = 0; j < NS; j++) hf = __builtin_fma(H[i][j], H[i][j], hf);
-  hnorm = __builtin_sqrt(hf);
-}
-
-// The same reduction on the LDS block sh = [H (row-major), Q^T K, Q^T L] of one workgroup (one
-// wave): each reflector is formed by every lane from the same LDS values, then lane j updates
-// column j of H (H <- P H) and lanes NS, NS + 1 update Q^T K and Q^T L, then lane i updates row i
-// (H <- H P).  O(NS^2) per reflector per lane instead of O(NS^3), and no matrix in registers.
+// The Householder reduction of A1 - I (delta form: its diagonal is 1 - O(ts) for a sampled plant)
+// on the LDS block sh = [H (row-major), Q^T K, Q^T L] of one workgroup (one wave).  Each reflector
+// is formed by every lane from the same LDS values; then lane j updates column j of H (H <- P H)
+// and lanes NS, NS + 1 update Q^T K and Q^T L; then lane i updates row i (H <- H P).  O(NS^2) work
+// per reflector and lane, and no matrix in registers.
 template <int NS>
 __device__ __forceinline__ void hess_prep_lds(const double* __restrict__ oa, const float* __restrict__ gg,
                                               double* sh, double& hnorm) {
