@@ -278,7 +278,11 @@ __device__ __forceinline__ bool hess_point(const double* __restrict__ sh, double
     const double ovr = sw ? vr : nv, ovi = sw ? vi : 0.0;
     const double p2 = ukr * ukr + uki * uki;
     pmin2 = __builtin_fmin(pmin2, p2);
-    const double inv = 1.0 / p2;
+    // 1 / p2 by the hardware reciprocal and two Newton steps (full double precision; a zero,
+    // infinite or NaN pivot gives NaN here, and the point is then reported not finite)
+    double inv = __builtin_amdgcn_rcp(p2);
+    inv = __builtin_fma(inv, __builtin_fma(-p2, inv, 1.0), inv);
+    inv = __builtin_fma(inv, __builtin_fma(-p2, inv, 1.0), inv);
     // l = o_k / u_k, y_k = kq'_k / u_k
     const double lr = (okr * ukr + oki * uki) * inv, li = (oki * ukr - okr * uki) * inv;
     const double ykr = (yr[k] * ukr + yi[k] * uki) * inv, yki = (yi[k] * ukr - yr[k] * uki) * inv;
