@@ -81,6 +81,20 @@ struct Gen {
         return kind == K_CONST ? cst(r, which, val) : (kind == K_STATE ? pos : prev);
       };
       const std::string a = opnd(in.akind, 0, in.aval), b = opnd(in.bkind, 1, in.bval);
+      // float form: SGN(x) followed by MUL(prev, x) on the same column is |x| (equal for every x
+      // but zero, where only the sign of the zero differs), a free source modifier of the next use
+      if (f32 && in.op == 6 && in.akind == K_STATE) {
+        int r2 = r + 1;
+        while (r2 < count && ops[r2].op == 0) r2++;
+        if (r2 < count && ops[r2].op == 3 && ops[r2].col == in.col &&
+            ((ops[r2].akind == K_PREV && ops[r2].bkind == K_STATE) ||
+             (ops[r2].akind == K_STATE && ops[r2].bkind == K_PREV))) {
+          o << "    " << res << " = __builtin_fabsf(" << pos << ");\n";
+          started = true;
+          r = r2;
+          continue;
+        }
+      }
       switch (in.op) {
         case 1: o << "    " << res << " = " << a << " + " << b << ";\n"; break;
         case 2: o << "    " << res << " = " << a << " - " << b << ";\n"; break;
