@@ -166,6 +166,23 @@ __device__ __forceinline__ void sweep_tail(const ProblemDev* __restrict__ P, int
                      (R)tab<double>(P, P->off_pabs)[w], (R)P->alpha, gr, gi, m, raw, b);
 }
 
+// |loop|, arg(loop) and b(w) in float for the mixed sweep's locator: the formulas of
+// sweep_tail_vals with each modulus as sqrt(x^2 + y^2) (no rescaling: a value beyond the float
+// range comes out infinite, and such a point is recomputed in FP64) and the quotient by the
+// hardware reciprocal; a few float ulps, within the locator's error allowance.
+__device__ __forceinline__ void locator_tail(float pr, float pim, float pabs, float alpha, float gr, float gi,
+                                             float& m, float& raw, float& b) {
+  const float rr = __builtin_fmaf(pr, gr, -(pim * gi)), ri = __builtin_fmaf(pr, gi, pim * gr);
+  m = __builtin_sqrtf(__builtin_fmaf(rr, rr, ri * ri));
+  raw = atan2f(ri, rr);
+  const float Ps = alpha * pabs;
+  const float ia = __builtin_amdgcn_rcpf(alpha);
+  const float Cs2 = __builtin_fmaf(gr, gr, gi * gi) * (ia * ia);
+  const float a1 = 1.0f + rr;
+  const float num = __builtin_sqrtf(__builtin_fmaf(a1, a1, ri * ri));
+  b = num * __builtin_amdgcn_rcpf(__builtin_sqrtf(__builtin_fmaf(Ps, Ps, 1.0f)) * __builtin_sqrtf(1.0f + Cs2));
+}
+
 // Sweep of the mixed chain on the Hessenberg form (the FP64 chain keeps the dense elimination
 // above for parity), in delta form: A1 - I = Q H Q^T by Householder reflectors, so
 //   G(z) = K^T (zI - A1)^-1 L = (Q^T K)^T ((z - 1) I - H)^-1 (Q^T L),
@@ -266,50 +283,57 @@ __device__ __forceinline__ bool hess_point(const double* __restrict__ sh, double
   }
   double vr = lq[0], vi = 0.0;
   double sr = 0.0, si = 0.0, y2 = 0.0, w2 = 0.0, pmin2 = __builtin_inf();
+  // (the products are written as explicit fused multiply-adds: these values locate points and
+  // carry their own error estimate, and the file is built without contraction)
+#define FMA __builtin_fma
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     // row k + 1 of M: -H[k+1][j], plus z - 1 on the diagonal; its column-k entry is real
     const double nk = (k + 1 < NS) ? -H(k + 1 < NS ? k + 1 : k, k) : 0.0;
-    const bool sw = (k + 1 < NS) && nk * nk > cr[k] * cr[k] + ci[k] * ci[k];
+    const bool sw = (k + 1 < NS) && nk * nk > FMA(cr[k], cr[k], ci[k] * ci[k]);
     const double ukr = sw ? nk : cr[k], uki = sw ? 0.0 : ci[k];
     const double okr = sw ? cr[k] : nk, oki = sw ? ci[k] : 0.0;
     const double nv = (k + 1 < NS) ? lq[k + 1 < NS ? k + 1 : k] : 0.0;
     const double wr = sw ? nv : vr, wi = sw ? 0.0 : vi;
     const double ovr = sw ? vr : nv, ovi = sw ? vi : 0.0;
-    const double p2 = ukr * ukr + uki * uki;
+    const double p2 = FMA(ukr, ukr, uki * uki);
     pmin2 = __builtin_fmin(pmin2, p2);
     // 1 / p2 by the hardware reciprocal and two Newton steps (full double precision; a zero,
     // infinite or NaN pivot gives NaN here, and the point is then reported not finite)
     double inv = __builtin_amdgcn_rcp(p2);
-    inv = __builtin_fma(inv, __builtin_fma(-p2, inv, 1.0), inv);
-    inv = __builtin_fma(inv, __builtin_fma(-p2, inv, 1.0), inv);
+    inv = FMA(inv, FMA(-p2, inv, 1.0), inv);
+    inv = FMA(inv, FMA(-p2, inv, 1.0), inv);
     // l = o_k / u_k, y_k = kq'_k / u_k
-    const double lr = (okr * ukr + oki * uki) * inv, li = (oki * ukr - okr * uki) * inv;
-    const double ykr = (yr[k] * ukr + yi[k] * uki) * inv, yki = (yi[k] * ukr - yr[k] * uki) * inv;
+    const double lr = FMA(okr, ukr, oki * uki) * inv, li = FMA(oki, ukr, -(okr * uki)) * inv;
+    const double ykr = FMA(yr[k], ukr, yi[k] * uki) * inv, yki = FMA(yi[k], ukr, -(yr[k] * uki)) * inv;
 #pragma unroll
     for (int j = k + 1; j < NS; j++) {
       const double njr = (j == k + 1 ? zmr : 0.0) - H(k + 1 < NS ? k + 1 : k, j), nji = (j == k + 1 ? zi : 0.0);
       const double ujr = sw ? njr : cr[j], uji = sw ? nji : ci[j];
       const double ojr = sw ? cr[j] : njr, oji = sw ? ci[j] : nji;
-      cr[j] = ojr - (lr * ujr - li * uji);
-      ci[j] = oji - (lr * uji + li * ujr);
-      yr[j] = yr[j] - (ujr * ykr - uji * yki);
-      yi[j] = yi[j] - (ujr * yki + uji * ykr);
+      // carried row o - l u, and kq' - u y_k
+      cr[j] = FMA(-lr, ujr, FMA(li, uji, ojr));
+      ci[j] = FMA(-lr, uji, FMA(-li, ujr, oji));
+      yr[j] = FMA(-ujr, ykr, FMA(uji, yki, yr[j]));
+      yi[j] = FMA(-ujr, yki, FMA(-uji, ykr, yi[j]));
     }
-    vr = ovr - (lr * wr - li * wi);
-    vi = ovi - (lr * wi + li * wr);
-    sr = sr + (ykr * wr - yki * wi);
-    si = si + (ykr * wi + yki * wr);
-    y2 = y2 + (ykr * ykr + yki * yki);
-    w2 = w2 + (wr * wr + wi * wi);
+    vr = FMA(-lr, wr, FMA(li, wi, ovr));
+    vi = FMA(-lr, wi, FMA(-li, wr, ovi));
+    sr = FMA(ykr, wr, FMA(-yki, wi, sr));
+    si = FMA(ykr, wi, FMA(yki, wr, si));
+    y2 = FMA(ykr, ykr, FMA(yki, yki, y2));
+    w2 = FMA(wr, wr, FMA(wi, wi, w2));
   }
-  const double g2 = sr * sr + si * si;
+  const double g2 = FMA(sr, sr, si * si);
+#undef FMA
   if (!(pmin2 > 0.0) || !__builtin_isfinite(g2)) return false;
-  // ||M|| <= sqrt(NS) |z - 1| + ||H||_F
-  const double mn = __builtin_sqrt((double)NS) * __builtin_sqrt(zmr * zmr + zi * zi) + hnorm;
-  const double e = (double)SWEEP_ERR_SAFETY * 1.1102230246251565e-16 * mn * __builtin_sqrt((y2 * w2) / (pmin2 * g2));
-  err = (e <= 1.0) ? (float)e + SWEEP_ERR_FLOOR : 4.0f;  // NaN and overflow: 4 (every guard then holds)
-  sweep_tail_vals<float>(pr, pim, pabs, alpha, (float)sr, (float)si, m, raw, b);
+  // the estimate needs a digit or two: float, with approximate reciprocals.  ||M|| <=
+  // sqrt(NS) |z - 1| + ||H||_F.  An overflow or a NaN gives 4 (every guard then holds).
+  const float mn = __builtin_sqrtf((float)NS) * __builtin_sqrtf((float)__builtin_fma(zmr, zmr, zi * zi)) + (float)hnorm;
+  const float ratio = (float)((y2 * w2) * __builtin_amdgcn_rcp(pmin2 * g2));
+  const float e = SWEEP_ERR_SAFETY * 1.1102230246251565e-16f * mn * __builtin_sqrtf(ratio);
+  err = (e <= 1.0f) ? e + SWEEP_ERR_FLOOR : 4.0f;
+  locator_tail(pr, pim, pabs, alpha, (float)sr, (float)si, m, raw, b);
   return true;
 #undef H
 }
