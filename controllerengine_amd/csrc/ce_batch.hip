@@ -490,7 +490,7 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   batch_mark(b, "fill", st);
   const ProblemDev* const* probs = (const ProblemDev* const*)b->ptrs.p;
   const StageBufs* bufs = (const StageBufs*)b->bufs.p;
-  // mixed: the nominal-loop eigensolve, float sweep and refinement run on the side stream, beside
+  // mixed: the nominal-loop eigensolve, the located sweep and the refinement run on the side stream, beside
   // the simulation and the fit (per-launch timing keeps everything on one stream)
   const bool fork = mixed && !b->profiling;
   if (fork && !b->side) {

@@ -67,7 +67,7 @@ struct ProblemDev {
   float fts, fref_amp;
   // byte offsets (from the start of this struct) of the per-frequency tables
   int64_t off_zr, off_zi, off_theta, off_pr, off_pi, off_pabs, off_pok;
-  int64_t off_zm1;  // Re(z) - 1 per frequency (double), for the float sweep's delta form
+  int64_t off_zm1;  // Re(z) - 1 per frequency (double), for the mixed sweep's delta form
   int64_t bytes_total;
 };
 

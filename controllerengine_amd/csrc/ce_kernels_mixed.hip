@@ -10,13 +10,16 @@
 //                  right-hand side sum_t phi(t)(y(t) - yss) runs over the stored trajectory with the
 //                  basis advanced by a double complex recurrence.  Then the same LDL^T solve and
 //                  score as the FP64 chain (src/modalsolve.cu).
-//   margin sweep   k_sweep_mark32: one wave per candidate, every frequency in float; the points
-//                  whose float values show (or come within a guard band of) a gain or phase crossing,
-//                  a non-finite value, or the smallest b(w) and its band are appended to a dense
-//                  list.  k_refine64: the listed points are recomputed with the FP64 sweep point (the
-//                  FP64 chain's arithmetic).  k_scan32: the reference's unwrap / crossing /
-//                  interpolation scan (src/allmargin.cu:119-180) in double over the refined values,
-//                  one wave per candidate (chunk-wise phase prefix, wave minima).
+//   margin sweep   k_sweep_mark32: one wave per candidate, every frequency located by an O(ns^2)
+//                  FP64 solve on the Hessenberg form with float |loop|, arg and b(w), each with an
+//                  error estimate; the points whose values show (or come within an error-widened
+//                  guard band of) a gain or phase crossing, a non-finite value, or the smallest
+//                  b(w) and its band are appended to a dense list.  k_refine_full64: the listed
+//                  points are recomputed with the FP64 sweep point (the FP64 chain's arithmetic),
+//                  and candidates with more than RMAX such points in full.  k_scan32: the
+//                  reference's unwrap / crossing / interpolation scan (src/allmargin.cu:119-180) in
+//                  double over the refined values, one wave per candidate (chunk-wise phase prefix,
+//                  wave minima).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -332,7 +335,8 @@ __device__ __forceinline__ V wave_excl_sum(V v, int lane) {
   return incl - v;
 }
 
-// Candidate c's float sweep is not trusted: append it to the FP64 list (one entry per candidate).
+// Candidate c has too many points to refine: append it to the list of candidates whose every
+// frequency is recomputed in FP64 (one entry per candidate).
 __device__ __forceinline__ void sweep_fallback(const StageBufs& B, int c, int lane) {
   if (lane == 0) {
     const int pos = atomicAdd(B.fcount, 1);
@@ -341,10 +345,10 @@ __device__ __forceinline__ void sweep_fallback(const StageBufs& B, int c, int la
   }
 }
 
-// Float sweep of every frequency, then the points whose float values show (or come within a guard
+// Located sweep of every frequency, then the points whose values show (or come within a guard
 // band of) a gain or phase crossing, a non-finite value, or the smallest b(w) (and those within a
 // band of it) are marked for FP64 refinement and appended to the dense refinement list.  The guard
-// bands are widened by each point's float error bound.  A candidate with more than RMAX marked
+// bands are widened by each point's error estimate.  A candidate with more than RMAX marked
 // points has its whole sweep recomputed in FP64 instead.
 template <int NS>
 __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
@@ -455,7 +459,7 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
     if (k < nk) lastPh = ph[k];
   const float prevPh0 = __shfl_up(lastPh, 1);
   // crossing marks: interval (w-1, w) flagged -> mark both ends.  Each guard is widened by the
-  // float error bound of the points involved (relative in |loop|, radians in the phase).
+  // error estimate of the points involved (relative in |loop|, radians in the phase).
   unsigned cross = 0;
   bool send_prev = false;
 #pragma unroll
@@ -909,7 +913,7 @@ static void launch_mark_refine_t(const ProblemDev* P, const StageBufs& B, hipStr
   kmark("k_refine_full64", st);
 }
 
-// float sweep + marking + FP64 refinement (independent of the simulation and the fit)
+// located sweep + marking + FP64 refinement (independent of the simulation and the fit)
 bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
   if (F > 64 * KMAX) return false;
   switch (ns) {

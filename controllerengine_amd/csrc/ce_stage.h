@@ -41,7 +41,7 @@ struct StageBufs {
   int32_t* list;                // n: candidates that reach the simulation (compacted)
   int32_t* count;               // 1: entries in list
   // mixed sweep
-  float* mr32;                  // n x F x 4: float |loop|, arg(loop), b(w), error bound
+  float* mr32;                  // n x F x 4: located |loop|, arg(loop), b(w), error estimate (float)
   int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite
   int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
   int32_t* rcount;              // 1
