@@ -84,3 +84,34 @@ def test_batch_autotune_and_update(engine, batch):
     assert np.array_equal(outs[1][1].view(np.uint64), s64.view(np.uint64))
     b.update(1, ps[1])
     b.set_options(use_graph=True, autotune=False)
+
+
+def order_jobs():
+    # serving-job plants of every order 2..8 in one batch (the batched sweep, simulation and eigen
+    # kernels of each order), each drawn from the smallest seed whose plant has that order
+    out = []
+    for ns in range(2, 9):
+        s = 0
+        while int(np.random.default_rng(s).integers(2, 9)) != ns:
+            s += 1
+        out.append(workloads.c5_job(np.random.default_rng(s), n=200, tmin=400, tmax=900, fmin=64, fmax=160))
+    return out
+
+
+@pytest.mark.parametrize("prec", [PRECISION_FP64, PRECISION_MIXED])
+def test_batch_every_order_equals_single(engine, prec):
+    from controllerengine_amd.batch import Batch
+    ps = order_jobs()
+    b = Batch(0)
+    try:
+        for p in ps:
+            b.add(p)
+        b.set_options(use_graph=True, autotune=False)
+        print("batch of orders", [p.ns for p in ps], "prec", prec, flush=True)
+        outs = b.evaluate(prec)
+        for p, (o32, o64) in zip(ps, outs):
+            s32, s64 = single(engine, p, prec)
+            assert np.array_equal(o64.view(np.uint64), s64.view(np.uint64)), (p.ns, p.name)
+            assert np.array_equal(o32.view(np.uint32), s32.view(np.uint32)), (p.ns, p.name)
+    finally:
+        b.close()
