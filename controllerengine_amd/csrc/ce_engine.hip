@@ -79,6 +79,10 @@ struct ce_engine {
   bool events_ok = false;
   hipStream_t last_stream = nullptr;
   JitCache* jit = nullptr;
+  // the specialised simulation kernel of the uploaded problem, per [mixed][gated] (NULL when its
+  // compilation failed); reset when a different problem is uploaded
+  void* sim_fn[2][2] = {};
+  bool sim_fn_known[2][2] = {};
   int last_sim_jit = 0;
   int last_mixed = 0;
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
@@ -111,6 +115,7 @@ static const ProblemDev* upload_problem(ce_engine* E, const std::vector<uint8_t>
   if (E->d_prob && E->blob == blob) return reinterpret_cast<const ProblemDev*>(E->d_prob);
   // the previous blob may still be read by queued kernels: drain before reusing the buffers
   HIPCHK(hipStreamSynchronize(st));
+  for (auto& row : E->sim_fn_known) row[0] = row[1] = false;
   if (E->d_prob_cap < blob.size()) {
     if (E->d_prob) HIPCHK(hipFree(E->d_prob));
     HIPCHK(hipMalloc(&E->d_prob, blob.size()));
@@ -254,9 +259,21 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   bool gated = false;
   if (jit_enabled()) {
     if (!E->jit) E->jit = jit_cache_create();
-    std::string jerr;
-    if (want_gate && jit_get(E->jit, gen_sim_source(H, mixed, true), &fn, jerr)) gated = true;
-    else if (!jit_get(E->jit, gen_sim_source(H, mixed, false), &fn, jerr)) fn = nullptr;
+    // resolved once per uploaded problem: generating the source to look it up takes tens of
+    // microseconds of host time, during which the device would wait for its next kernel
+    auto lookup = [&](bool g) -> void* {
+      const int mi = mixed ? 1 : 0, gi = g ? 1 : 0;
+      if (!E->sim_fn_known[mi][gi]) {
+        std::string jerr;
+        void* f = nullptr;
+        if (!jit_get(E->jit, gen_sim_source(H, mixed, g), &f, jerr)) f = nullptr;
+        E->sim_fn[mi][gi] = f;
+        E->sim_fn_known[mi][gi] = true;
+      }
+      return E->sim_fn[mi][gi];
+    };
+    if (want_gate && (fn = lookup(true)) != nullptr) gated = true;
+    else fn = lookup(false);
   }
   // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
   MarkerScope marks(E, st);
