@@ -229,13 +229,6 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     if (B.d_ata) launch_fill_i32((int32_t*)B.d_ata, 0, nn * 4 * nt * nt, st);
     if (B.d_atb) launch_fill_i32((int32_t*)B.d_atb, 0, nn * 2 * nt, st);
   }
-  launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
-  launch_fill_i32(B.sweep_fail, 0, nn, st);
-  launch_fill_i32(B.count, 0, 1, st);
-  if (mixed) {
-    launch_fill_i32(B.rcount, 0, 1, st);
-    launch_fill_i32(B.fcount, 0, 1, st);
-  }
   launch_info_init(d_info_in, B.info, nn * H.info_w, st);
   E->last_stream = st;
 
@@ -283,6 +276,15 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     launch_fill_i32((int*)E->gate.p, 0, 1, st);
   }
   launch_eig_lin(H.nt, P, B, st);
+  // state the later stages start from, enqueued after the first long kernel so that it starts as
+  // early as possible
+  launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
+  launch_fill_i32(B.sweep_fail, 0, nn, st);
+  launch_fill_i32(B.count, 0, 1, st);
+  if (mixed) {
+    launch_fill_i32(B.rcount, 0, 1, st);
+    launch_fill_i32(B.fcount, 0, 1, st);
+  }
   launch_compact(P, B, st);
   // the nominal eigensolve walks the compacted list: after the compaction, on this stream or the
   // second one
