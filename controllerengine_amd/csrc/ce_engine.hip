@@ -23,7 +23,7 @@ void launch_eig_nom(int ns, const ProblemDev* P, const StageBufs& B, hipStream_t
 bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_scan32(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_scan32_split(const ProblemDev* P, const StageBufs& B, hipStream_t st);
-void launch_margin_combine(const ProblemDev* P, const StageBufs& B, hipStream_t st);
+void launch_margin_combine_final(const ProblemDev* P, const StageBufs& B, float* info_out, hipStream_t st);
 bool launch_sim_fp64(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_fit_fp64(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
@@ -325,14 +325,17 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   HIPCHK(hipEventRecord(E->ev[3], st));
   // stage 4: margin sweep, crossings and total score
   if (fork) {
+    // the scan's row updates and the float copy of the rows in one launch
     HIPCHK(hipStreamWaitEvent(st, E->ev_join, 0));
-    launch_margin_combine(P, B, st);
-  } else if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st))) {
-    throw std::runtime_error("no sweep kernel for these dimensions");
+    launch_margin_combine_final(P, B, d_info_out, st);
+    HIPCHK(hipEventRecord(E->ev[4], st));
+  } else {
+    if (!(mixed ? launch_sweep_mixed(ns, F, P, B, st) : launch_sweep_fp64(ns, F, P, B, st)))
+      throw std::runtime_error("no sweep kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev[4], st));
+    launch_info_final(B.info, d_info_out, nn * H.info_w, st);
   }
-  HIPCHK(hipEventRecord(E->ev[4], st));
   HIPCHK(hipGetLastError());
-  launch_info_final(B.info, d_info_out, nn * H.info_w, st);
   if (d_info64_out)
     HIPCHK(hipMemcpyAsync(d_info64_out, B.info, nn * H.info_w * sizeof(double), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipEventRecord(E->ev[5], st));

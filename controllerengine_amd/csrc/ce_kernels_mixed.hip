@@ -841,9 +841,7 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
 
 // After the fit: the row updates of the crossing scan from its stored minima (lane per candidate);
 // the same branches, in the same order, as the unsplit scan.
-__global__ __launch_bounds__(256) void k_margin_combine(const ProblemDev* __restrict__ P, StageBufs B) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= B.n) return;
+__device__ __forceinline__ void margin_combine_row(const ProblemDev* __restrict__ P, const StageBufs& B, int c) {
   double* row = B.info + (size_t)c * P->info_w;
   double* sc = row + P->info_scoring;
   double* am = row + P->info_allmargin;
@@ -868,6 +866,19 @@ __global__ __launch_bounds__(256) void k_margin_combine(const ProblemDev* __rest
     return;
   }
   margin_score(P, sc, am, r[0], r[1], r[2], r[3], r[4] != 0.0);
+}
+
+// The row updates, then the float copy of the workgroup's rows (the INFO rows the caller gets):
+// one launch at the end of the two-stream chain.
+__global__ __launch_bounds__(256) void k_margin_combine_final(const ProblemDev* __restrict__ P, StageBufs B,
+                                                              float* __restrict__ out) {
+  const int c0 = blockIdx.x * 256;
+  const int c = c0 + threadIdx.x;
+  if (c < B.n) margin_combine_row(P, B, c);
+  __syncthreads();
+  const size_t w = (size_t)P->info_w;
+  const size_t e0 = (size_t)c0 * w, e1 = (size_t)(c0 + 256 < B.n ? c0 + 256 : B.n) * w;
+  for (size_t i = e0 + threadIdx.x; i < e1; i += 256) out[i] = (float)B.info[i];
 }
 
 __global__ __launch_bounds__(64) void k_scan32(const ProblemDev* __restrict__ P, StageBufs B) {
@@ -956,9 +967,9 @@ void launch_scan32_split(const ProblemDev* P, const StageBufs& B, hipStream_t st
   hipLaunchKernelGGL(k_scan32_split, dim3(B.n), dim3(64), 0, st, P, B);
   kmark("k_scan32", st);
 }
-void launch_margin_combine(const ProblemDev* P, const StageBufs& B, hipStream_t st) {
-  hipLaunchKernelGGL(k_margin_combine, dim3((B.n + 255) / 256), dim3(256), 0, st, P, B);
-  kmark("k_margin_combine", st);
+void launch_margin_combine_final(const ProblemDev* P, const StageBufs& B, float* info_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_margin_combine_final, dim3((B.n + 255) / 256), dim3(256), 0, st, P, B, info_out);
+  kmark("k_margin_combine_final", st);
 }
 
 bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
