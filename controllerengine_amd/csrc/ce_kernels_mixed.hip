@@ -619,21 +619,11 @@ __device__ __forceinline__ void k_full64_body(const ProblemDev* __restrict__ P, 
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_full64(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_full64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
-}
-
-template <int NS>
 __global__ __launch_bounds__(64) void k_full64_batch(BatchTab T) {
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
     k_full64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
-}
-
-template <int NS>
-__global__ __launch_bounds__(64) void k_refine64(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_refine64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 // The point refinement (first REFINE_BLOCKS workgroups) and the whole-candidate FP64 sweeps (the
@@ -674,7 +664,7 @@ __device__ __forceinline__ void margin_score(const ProblemDev* __restrict__ P, d
 // place, one wave per candidate: each interval's contributions are independent once the phase is
 // unwrapped (chunk-wise prefix in double); the minima and flags reduce over the wave.
 // SPLIT: the scan runs beside the fit (second stream), so the behaviour score is not final yet: it
-// only stores its minima in B.mres, for k_margin_combine to apply after the fit.
+// only stores its minima in B.mres, for k_margin_combine_final to apply after the fit.
 template <bool SPLIT>
 __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int c = bx;
