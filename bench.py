@@ -226,8 +226,8 @@ def bench_c5(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--n", type=int, default=0, help="candidates (default: the configuration's count)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
