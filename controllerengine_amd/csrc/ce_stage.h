@@ -67,7 +67,8 @@ struct StageBufs {
 // wave halves the waves in flight (c3 LIN_A eigensolve 7.8 vs 8.0 ms).  At order 8, 4 lanes with
 // two rows each measured faster than 8 lanes (c4 order-8 LIN_A eigensolve 2.07 vs 2.55 ms; c2
 // unchanged): every lane of a group evaluates the shifts and Householder coefficients, so fewer
-// lanes per candidate do less redundant work.
+// lanes per candidate do less redundant work.  2 lanes with four rows each went too far (c2 LIN_A
+// eigensolve 0.49 vs 0.38 ms: twice the LDS per wave, half the waves in flight).
 constexpr int EIG16_G = 16;
 constexpr int EIG8_G = 4;
 inline int eig_key(int order) { return order <= 4 ? 4 : (order <= 8 ? 8 : 16); }
