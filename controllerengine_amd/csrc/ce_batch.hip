@@ -362,9 +362,14 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   group(BK_COMPACT, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_compact");
   // the nominal eigensolve walks each problem's compacted list, so it follows the compaction
   group(BK_EIG_NOM, [&](const PP& p) { return eig_key(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(2 * p.ns)); }, "k_eig_nom");
-  {  // simulation: one group per structure
+  {  // simulation: one group per structure, its problems longest horizon first (a wave's work is
+     // proportional to the horizon: the long ones start first instead of forming the tail)
+    std::vector<int> order(J);
+    for (int j = 0; j < J; j++) order[j] = j;
+    auto horizon = [&](int j) { return reinterpret_cast<const ProblemDev*>(b->P[j].blob.data())->T; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return horizon(x) > horizon(y); });
     std::map<std::string, Launch> g;
-    for (int j = 0; j < J; j++) {
+    for (const int j : order) {
       const ProblemDev& H = *reinterpret_cast<const ProblemDev*>(b->P[j].blob.data());
       const std::string k = sim_structure_key(H, mixed);
       Launch& L = g[k];
