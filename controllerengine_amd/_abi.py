@@ -158,6 +158,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libControllerEngine.so is not built: run python controllerengine_amd/build.py "
                                "(or __graft_entry__.build())")
+        # torch carries its own copy of the HIP runtime.  Loaded first, it provides the
+        # libamdhip64 this library links against; loaded after this library, it would bring a
+        # second runtime into the process, and whichever initialises second finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
 

@@ -288,4 +288,18 @@ CE_HD double hypot(double x, double y) {
   return sqrt_(fma_(ax, ax, ay * ay));
 }
 
+// Single-precision logf / expf / sinf / cosf / atan2f: the double functions above rounded to float.
+// Their double results lie within a few double ulp of the exact value, so the rounded float is the
+// correctly rounded one except when the exact value falls within that distance of a midpoint
+// between two floats.
+CE_HD float logf_(float x) { return (float)log((double)x); }
+CE_HD float expf_(float x) { return (float)exp((double)x); }
+CE_HD void sincosf_(float x, float& s, float& c) {
+  double sd, cd;
+  sincos((double)x, sd, cd);
+  s = (float)sd;
+  c = (float)cd;
+}
+CE_HD float atan2f_(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 }  // namespace cedm

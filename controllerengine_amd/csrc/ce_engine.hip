@@ -32,6 +32,7 @@ void launch_info_final(const double* in, float* out, size_t count, hipStream_t s
 bool launch_sim_mixed(int ns, int nd, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_fit_mixed(int nt, const ProblemDev* P, const StageBufs& B, hipStream_t st);
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+bool mixed_sweep_fits(int F);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_gate_wait(const int* gate, int gen, hipStream_t st);
@@ -233,7 +234,8 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   E->last_stream = st;
 
   // with per-kernel timing on, everything stays on one stream so that the events bracket kernels
-  const bool fork = mixed && !E->profiling;
+  // (the second stream runs the located sweep, which a grid of more than 512 points goes without)
+  const bool fork = mixed && !E->profiling && mixed_sweep_fits(F);
   if (fork && !E->side) {
     HIPCHK(hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));

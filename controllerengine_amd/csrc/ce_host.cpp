@@ -470,20 +470,22 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   double* pab = reinterpret_cast<double*>(blob.data() + P.off_pabs);
   int32_t* pok = reinterpret_cast<int32_t*>(blob.data() + P.off_pok);
   double* zm1 = reinterpret_cast<double*>(blob.data() + P.off_zm1);
-  // log-spaced grid from freq_min to Nyquist (reference src/main.cu:421-428), evaluated in double
-  const double lmin = cedm::log((double)c.freq_min), lmax = cedm::log((double)d.freq_max);
-  const double inc = (lmax - lmin) / (double)(F - 1);
+  // log-spaced grid from freq_min to Nyquist, derived in float32 as the reference does
+  // (src/main.cu:423-427: logf/expf spacing, z = (cosf, sinf) of the float angle) and promoted;
+  // theta_w = arg(z) of the float point (src/allmargin.cu:111)
+  const float lminf = cedm::logf_(c.freq_min), lmaxf = cedm::logf_(d.freq_max);
+  const float incf = (lmaxf - lminf) / (float)(F - 1);
   for (int w = 0; w < F; w++) {
-    const double theta = cedm::exp(lmin + inc * (double)w) * (double)d.ts;
-    double s, co;
-    cedm::sincos(theta, s, co);
+    const float thf = cedm::expf_(lminf + incf * (float)w) * d.ts;
+    float sf, cf;
+    cedm::sincosf_(thf, sf, cf);
+    const double co = cf, s = sf;
     zr[w] = co;
     zi[w] = s;
-    // Re(z) - 1 = -2 sin^2(theta / 2), without the cancellation of co - 1 at low frequencies
-    double sh, chh;
-    cedm::sincos(0.5 * theta, sh, chh);
-    zm1[w] = -2.0 * sh * sh;
-    th[w] = cedm::atan2(s, co);
+    // Re(z) - 1: exact in double for a float Re(z) in [0.5, 1] (Sterbenz), and the difference of
+    // two doubles otherwise
+    zm1[w] = co - 1.0;
+    th[w] = (double)cedm::atan2f_(sf, cf);
     double xr[MAXS], xi[MAXS];
     const bool ok = shifted_solve_host(P.pa, ns, co, s, P.pb, xr, xi);
     pok[w] = ok ? 1 : 0;
@@ -498,10 +500,10 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
     }
     pab[w] = cedm::hypot(pr[w], pim[w]);
   }
-  {  // alpha: reference src/allmargin.cu:32-41
-    const double ang = (double)d.crossover * (double)d.ts;
-    double s, co;
-    cedm::sincos(ang, s, co);
+  {  // alpha: reference src/allmargin.cu:32-41, at z = polar(1, crossover * ts) in float
+    float sf, cf;
+    cedm::sincosf_(d.crossover * d.ts, sf, cf);
+    const double s = sf, co = cf;
     double xr[MAXS], xi[MAXS];
     const bool ok = shifted_solve_host(P.pa, ns, co, s, P.pb, xr, xi);
     double gr, gi;

@@ -994,7 +994,14 @@ bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStrea
   return false;
 }
 
+bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);
+
+// The located float sweep holds KMAX points per lane of a wave; a longer grid is swept by the FP64
+// chain's kernels (every point in FP64, the same rows as the FP64 chain's sweep).
+bool mixed_sweep_fits(int F) { return F <= 64 * KMAX; }
+
 bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st) {
+  if (!mixed_sweep_fits(F)) return launch_sweep_fp64(ns, F, P, B, st);
   if (!launch_mark_refine(ns, F, P, B, st)) return false;
   launch_scan32(P, B, st);
   return true;

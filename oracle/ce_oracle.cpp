@@ -642,18 +642,19 @@ bool setup(const ce_config& c, const ce_plant& pl, Problem& P, std::string& err)
   // reference vector (main.cu:414): double compare against ref_step_time*time
   P.ref.resize(c.time);
   for (int t = 0; t < c.time; t++) P.ref[t] = ((double)t < c.ref_step_time * (double)c.time) ? 0.0 : (double)c.ref_amplitude;
-  // frequency grid: the reference's log spacing from freq_min to Nyquist (main.cu:421-428), evaluated in double
+  // frequency grid (main.cu:423-427), float32 as written: increment = (logf(fmax) - logf(fmin)) /
+  // (F - 1), theta_i = expf(logf(fmin) + increment * i) * ts, z_i = (cosf theta_i, sinf theta_i);
+  // promoted to double.  theta_w = arg(z_w) taken in float (allmargin.cu:111).
   const int F = c.freq;
-  const double lmin = cedm::log((double)c.freq_min), lmax = cedm::log((double)P.d.freq_max);
-  const double inc = (lmax - lmin) / (double)(F - 1);
+  const float inc = (cedm::logf_(P.d.freq_max) - cedm::logf_(c.freq_min)) / (float)(F - 1);
   P.z.resize(F);
   P.theta.resize(F);
   for (int w = 0; w < F; w++) {
-    const double th = cedm::exp(lmin + inc * (double)w) * (double)P.d.ts;
-    double s, co;
-    cedm::sincos(th, s, co);
-    P.z[w] = Cx{co, s};
-    P.theta[w] = cedm::atan2(s, co);
+    const float th = cedm::expf_(cedm::logf_(c.freq_min) + inc * (float)w) * P.d.ts;
+    float s, co;
+    cedm::sincosf_(th, s, co);
+    P.z[w] = Cx{(double)co, (double)s};
+    P.theta[w] = (double)cedm::atan2f_(s, co);
   }
   // margins plant
   if (c.serial) {
@@ -678,12 +679,11 @@ bool setup(const ce_config& c, const ce_plant& pl, Problem& P, std::string& err)
       P.presp[w] = Cx{0.0, 0.0};
     }
   }
-  {  // alpha: allmargin.cu:32-41
-    const double ang = (double)P.d.crossover * (double)P.d.ts;
-    double s, co;
-    cedm::sincos(ang, s, co);
+  {  // alpha: allmargin.cu:32-41; zc = std::polar(1.0f, crossover * ts) in float
+    float s, co;
+    cedm::sincosf_(P.d.crossover * P.d.ts, s, co);
     Cx x[CE_MAX_STATESPACE_DIM];
-    const bool ok = shifted_solve(P.pa, ns, Cx{co, s}, P.pb, x);
+    const bool ok = shifted_solve(P.pa, ns, Cx{(double)co, (double)s}, P.pb, x);
     Cx g = rdot(P.pc, x, ns);
     g.r = g.r + P.pd;
     const double pmag = cedm::hypot(g.r, g.i);

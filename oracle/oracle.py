@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libce_oracle.so")
+# CE_ORACLE_LIB names another build of the oracle (tests/test_oracle_mutants.py loads deliberately
+# broken copies through it to show that the pinning tests catch each mistake)
+LIB_PATH = os.environ.get("CE_ORACLE_LIB") or os.path.join(HERE, "libce_oracle.so")
 _lib = None
 
 

@@ -37,3 +37,41 @@ def linear_problem(A_I, B, C, gains, T=200, sat=None, F=64, fit=(0.05, 0.75), st
                    base_params=P.array()[None], dc_info=np.array(dc, np.int32), u_info=np.array(u, np.int32),
                    nl_info=np.zeros((0, n + 1), np.int32), gains=g,
                    info=np.zeros((g.shape[0], info_width(cfg)), np.float32))
+
+
+def general_problem(sa_i, sb, sc, sd, da_i, dbu, dc_col, gains, T=400, F=64, fit=(0.05, 0.75), steady=0.75,
+                    ref_step=0.05, dt=1e-3, amplitude=1.0, bandwidth=1.0, spec=None, info=None, sa=None, da=None):
+    """Observer model (STATESPACE_*: A_I, B, C, D) and plant (DISCRETE_*: A_I, BU, C = unit vector on
+    column dc_col) given separately; y = x_plant[dc_col] by a DC MUL(state, 1) instruction and a
+    pass-through U chain.  A = I + A_I unless sa / da give A explicitly."""
+    sa_i = np.atleast_2d(np.asarray(sa_i, dtype=np.float64))
+    da_i = np.atleast_2d(np.asarray(da_i, dtype=np.float64))
+    ns, nd = sa_i.shape[0], da_i.shape[0]
+    sa = np.eye(ns) + sa_i if sa is None else np.atleast_2d(np.asarray(sa, dtype=np.float64))
+    da = np.eye(nd) + da_i if da is None else np.atleast_2d(np.asarray(da, dtype=np.float64))
+    dc = np.zeros(nd)
+    dc[dc_col] = 1.0
+    P = Params()
+    one = P.const(1.0)
+    dcr = [row(nd, dc_col, OP_MUL, STATE, one)]
+    u = [row(ns, 0, OP_MUL, PREV, one)]
+    sp = dict(SPEC)
+    sp.update(spec or {})
+    cfg = dict(statespace_dim=ns, discrete_dim=nd, nonlin_dim=0, n_params=len(P.vals), input_size=0,
+               dc_instructions=1, u_instructions=1, nl_instructions=[], n_cores=128, n_rounds=1, mode="batch",
+               time=T, duration=T * dt, ref_step_time=ref_step, ref_amplitude=amplitude, fit_start=fit[0],
+               fit_end=fit[1], steady_start=steady, bandwidth=bandwidth, freq=F, freq_min=1e-2,
+               dump_diagnostics=False, **sp)
+    g = np.asarray(gains, dtype=np.float32).reshape(-1, 2 * ns + 1)
+    inf = np.zeros((g.shape[0], info_width(cfg)), np.float32) if info is None else np.asarray(info, np.float32)
+    return Problem(config=cfg, ss_a=np.float32(sa), ss_a_i=np.float32(sa_i), ss_b=np.float32(np.ravel(sb)),
+                   ss_c=np.float32(np.ravel(sc)), ss_d=np.float32([sd]), d_a=np.float32(da)[None],
+                   d_a_i=np.float32(da_i)[None], d_bu=np.float32(np.ravel(dbu))[None],
+                   d_bw=np.zeros((1, nd, 0), np.float32), d_c=np.float32(dc)[None], base_params=P.array()[None],
+                   dc_info=np.array(dcr, np.int32), u_info=np.array(u, np.int32),
+                   nl_info=np.zeros((0, nd + 1), np.int32), gains=g, info=inf)
+
+
+def f32(x):
+    """Values as the engine reads them: rounded to float32, then promoted."""
+    return np.float32(x).astype(np.float64)
