@@ -10,7 +10,9 @@
 //
 // Two variants are generated:
 //   fp64   double state; stores the fit window of y (SoA) and yss/overshoot (the FP64 chain).
-//   mixed  float state; stores the fit window of y as float (the mixed chain).
+//   mixed  float state kept as hi/lo pairs with the reference's compensated update (TwoSum then
+//          Fast2Sum, src/main.cu:129-155; the lo parts enter through dot products of their own,
+//          src/discrete_solves.cu:35-77); stores the fit window of y as float (the mixed chain).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -169,6 +171,10 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
   o << "  const " << T << " N = (" << T << ")g[" << 2 * ns << "];\n";
   for (int i = 0; i < ns; i++) o << "  " << T << " xh" << i << " = " << g.zero() << ";\n";
   for (int i = 0; i < nd; i++) o << "  " << T << " xp" << i << " = " << g.zero() << ";\n";
+  if (mixed) {  // lo parts of the compensated state
+    for (int i = 0; i < ns; i++) o << "  float xhl" << i << " = 0.0f;\n";
+    for (int i = 0; i < nd; i++) o << "  float xpl" << i << " = 0.0f;\n";
+  }
   o << "  " << T << " ysum = " << g.zero() << ", ymax = -" << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ", ymin = "
     << (mixed ? "__builtin_inff()" : "__builtin_inf()") << ";\n";
   o << "  " << T << "* yb = (" << T << "*)A.ybuf + idx;\n";
@@ -204,6 +210,8 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
   o << "    ymax = " << g.fmax() << "(ymax, y); ymin = " << g.fmin() << "(ymin, y);\n";
   o << "    " << T << " kx = " << g.zero() << ";\n";
   for (int k = 0; k < ns; k++) o << "    kx = " << g.fma() << "(K" << k << ", xh" << k << ", kx);\n";
+  if (mixed)
+    for (int k = 0; k < ns; k++) o << "    kx = " << g.fma() << "(K" << k << ", xhl" << k << ", kx);\n";
   o << "    const " << T << " rr = RZ ? " << g.zero() << " : amp;\n";
   o << "    const " << T << " u_raw = " << g.fma() << "(N, rr, -kx);\n";
   const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", ins_cst(P.u_ops, CE_OFF(u_ops), CE_OFF(fu_aval), CE_OFF(fu_bval)));
@@ -231,6 +239,8 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
     std::vector<std::pair<std::string, std::string>> tc;
     for (int k = 0; k < ns; k++)
       if ((float)P.sc[k] != 0.0f) tc.push_back({K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4), "xh" + std::to_string(k)});
+    for (int k = 0; k < ns; k++)
+      if ((float)P.sc[k] != 0.0f) tc.push_back({K(P.sc[k], CE_OFF(sc) + k * 8, CE_OFF(fsc) + k * 4), "xhl" + std::to_string(k)});
     if ((float)P.sd != 0.0f) tc.push_back({K(P.sd, CE_OFF(sd), CE_OFF(fsd)), "uu"});
     acc("cx", tc, g.zero());
     o << "    const " << T << " yerr = y - cx;\n";
@@ -239,14 +249,21 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
       const std::string init = (float)P.sb[i] != 0.0f
           ? g.fma() + "(L" + std::to_string(i) + ", yerr, " + K(P.sb[i], CE_OFF(sb) + i * 8, CE_OFF(fsb) + i * 4) + " * uu)"
           : "L" + std::to_string(i) + " * yerr";
-      std::vector<std::pair<std::string, std::string>> td;
+      std::vector<std::pair<std::string, std::string>> td, tl;
       for (int k = 0; k < ns; k++)
-        if ((float)P.sai[i * ns + k] != 0.0f)
-          td.push_back({K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4), "xh" + std::to_string(k)});
+        if ((float)P.sai[i * ns + k] != 0.0f) {
+          const std::string cst = K(P.sai[i * ns + k], CE_OFF(sai) + (i * ns + k) * 8, CE_OFF(fsai) + (i * ns + k) * 4);
+          td.push_back({cst, "xh" + std::to_string(k)});
+          tl.push_back({cst, "xhl" + std::to_string(k)});
+        }
       acc("dh" + std::to_string(i), td, init);
+      acc("dl" + std::to_string(i), tl, g.zero());
     }
   }
-  for (int i = 0; i < ns; i++) o << "    xh" << i << " = xh" << i << " + dh" << i << ";\n";
+  for (int i = 0; i < ns; i++) {
+    if (mixed) o << "    ce_hladd(xh" << i << ", xhl" << i << ", dh" << i << ", dl" << i << ");\n";
+    else o << "    xh" << i << " = xh" << i << " + dh" << i << ";\n";
+  }
   snl = sxp;
   snl.push_back("y");
   std::vector<std::string> W;
@@ -267,15 +284,20 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
           << ", xp" << k << ", ph" << i << ");\n";
     } else {
       o << "    " << T << " ph" << i << " = " << g.zero() << ";\n";
-      auto term = [&](double v, size_t od, size_t of, const std::string& x) {
-        if ((float)v != 0.0f) o << "    ph" << i << " = " << g.fma() << "(" << K(v, od, of) << ", " << x << ", ph" << i << ");\n";
+      o << "    " << T << " pl" << i << " = " << g.zero() << ";\n";
+      auto term = [&](const char* acc, double v, size_t od, size_t of, const std::string& x) {
+        if ((float)v != 0.0f) o << "    " << acc << i << " = " << g.fma() << "(" << K(v, od, of) << ", " << x << ", " << acc << i << ");\n";
       };
-      term(P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4, "uu");
-      for (int q = 0; q < nw; q++) term(P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4, W[q]);
-      for (int k = 0; k < nd; k++) term(P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4, "xp" + std::to_string(k));
+      term("ph", P.dbu[i], CE_OFF(dbu) + i * 8, CE_OFF(fdbu) + i * 4, "uu");
+      for (int q = 0; q < nw; q++) term("ph", P.dbw[i * nw + q], CE_OFF(dbw) + (i * nw + q) * 8, CE_OFF(fdbw) + (i * nw + q) * 4, W[q]);
+      for (int k = 0; k < nd; k++) term("ph", P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4, "xp" + std::to_string(k));
+      for (int k = 0; k < nd; k++) term("pl", P.dai[i * nd + k], CE_OFF(dai) + (i * nd + k) * 8, CE_OFF(fdai) + (i * nd + k) * 4, "xpl" + std::to_string(k));
     }
   }
-  for (int i = 0; i < nd; i++) o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
+  for (int i = 0; i < nd; i++) {
+    if (mixed) o << "    ce_hladd(xp" << i << ", xpl" << i << ", ph" << i << ", pl" << i << ");\n";
+    else o << "    xp" << i << " = xp" << i << " + ph" << i << ";\n";
+  }
   o << "  };\n";
   in_loop = false;
   if (structural && hoist) {
@@ -328,7 +350,11 @@ static const char* kSimArgs =
     "static __device__ inline float ce_maxf(float a, float b) { float r; asm(\"v_max_f32 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
     "static __device__ inline float ce_minf(float a, float b) { float r; asm(\"v_min_f32 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
     "static __device__ inline double ce_max(double a, double b) { double r; asm(\"v_max_f64 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
-    "static __device__ inline double ce_min(double a, double b) { double r; asm(\"v_min_f64 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n";
+    "static __device__ inline double ce_min(double a, double b) { double r; asm(\"v_min_f64 %0, %1, %2\" : \"=v\"(r) : \"v\"(a), \"v\"(b)); return r; }\n"
+    // compensated state update (HI, LO) += (h, l): TwoSum, then Fast2Sum
+    "static __device__ inline __attribute__((always_inline)) void ce_hladd(float& H, float& Lo, float h, float l) {\n"
+    "  const float sh = H + h; const float b0 = sh - H; const float sl = (H - (sh - b0)) + (h - b0);\n"
+    "  const float v = (Lo + l) + sl; H = sh + v; Lo = v - (H - sh); }\n";
 
 // Both forms raise the simulation waves' issue priority (s_setprio 3): the simulation is one serial
 // recurrence per candidate with a single wave on each SIMD it uses, and the kernels that run beside

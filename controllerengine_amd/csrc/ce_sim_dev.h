@@ -3,6 +3,12 @@
 // plant dimensions.  One candidate per lane; the state lives in VGPRs; plant and observer matrices
 // and the decoded instruction chains are wave-uniform (scalar loads).  Used when runtime
 // specialisation is disabled and for batches of uneven problems.
+//
+// The float form keeps the reference's compensated state: each state is a hi/lo pair, the lo parts
+// enter K x, C x and the increments through dot products of their own, and a step adds its
+// increment pair with TwoSum followed by Fast2Sum (src/main.cu:129-155 hladdf,
+// src/discrete_solves.cu:35-77).  The instruction chains read the hi parts only, as the reference's
+// do.  The FP64 form needs no pairs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,6 +55,17 @@ struct RealOps<float> {
   __device__ static const float* dbw(const ProblemDev* P) { return P->fdbw; }
   __device__ static float amp(const ProblemDev* P) { return P->fref_amp; }
 };
+
+// (HI, LO) += (h, l): TwoSum of the hi parts, the lo parts and the TwoSum error added, then Fast2Sum
+// (reference src/main.cu:129-155).  Every operation is a separate rounding (no contraction).
+__device__ __forceinline__ void hl_add(float& HI, float& LO, float h, float l) {
+  const float sh = HI + h;
+  const float b0 = sh - HI;
+  const float sl = (HI - (sh - b0)) + (h - b0);
+  const float v = (LO + l) + sl;
+  HI = sh + v;
+  LO = v - (HI - sh);
+}
 
 // Instruction chain (reference src/main.cu:260-291).  ops/count/col are wave-uniform.
 template <typename R, int DIM, int MAXCOUNT, bool HAS_SEED>
@@ -105,11 +122,12 @@ __device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, con
 #pragma unroll
   for (int i = 0; i < NS; i++) { K[i] = (R)g[i]; L[i] = (R)g[NS + i]; }
   const R N = (R)g[2 * NS];
-  R xh[NS], xp[ND];
+  constexpr bool HL = sizeof(R) == 4;  // compensated float state
+  R xh[NS], xp[ND], xhl[NS], xpl[ND];
 #pragma unroll
-  for (int i = 0; i < NS; i++) xh[i] = R(0);
+  for (int i = 0; i < NS; i++) xh[i] = xhl[i] = R(0);
 #pragma unroll
-  for (int i = 0; i < ND; i++) xp[i] = R(0);
+  for (int i = 0; i < ND; i++) xp[i] = xpl[i] = R(0);
   const int T = P->T, f0 = P->fit_start_id, f1 = P->fit_start_id + P->fit_count;
   const int s0 = P->steady_start_id, k0 = P->ref_step_id, nw = P->nw;
   const int dc_n = P->dc_n, u_n = P->u_n;
@@ -133,23 +151,40 @@ __device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, con
     R kx = R(0);
 #pragma unroll
     for (int k = 0; k < NS; k++) kx = O::fma(K[k], xh[k], kx);
+    if (HL) {
+#pragma unroll
+      for (int k = 0; k < NS; k++) kx = O::fma(K[k], xhl[k], kx);
+    }
     const R r = (t < k0) ? R(0) : amp;
     const R u_raw = O::fma(N, r, -kx);
     const R u = interp_dev<R, NS, CE_MAX_U_INSTRUCTIONS, true>(P->u_ops, u_n, xh, u_raw);
     R cx = R(0);
 #pragma unroll
     for (int k = 0; k < NS; k++) cx = O::fma(scv[k], xh[k], cx);
+    if (HL) {
+#pragma unroll
+      for (int k = 0; k < NS; k++) cx = O::fma(scv[k], xhl[k], cx);
+    }
     const R yerr = y - O::fma(sd, u, cx);
-    R dh[NS];
+    R dh[NS], dl[NS];
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       R acc = O::fma(L[i], yerr, sb[i] * u);
 #pragma unroll
       for (int k = 0; k < NS; k++) acc = O::fma(sai[i * NS + k], xh[k], acc);
       dh[i] = acc;
+      R lo = R(0);
+      if (HL) {
+#pragma unroll
+        for (int k = 0; k < NS; k++) lo = O::fma(sai[i * NS + k], xhl[k], lo);
+      }
+      dl[i] = lo;
     }
 #pragma unroll
-    for (int i = 0; i < NS; i++) xh[i] = xh[i] + dh[i];
+    for (int i = 0; i < NS; i++) {
+      if constexpr (HL) hl_add(xh[i], xhl[i], dh[i], dl[i]);
+      else xh[i] = xh[i] + dh[i];
+    }
     R nls[ND + 1];
 #pragma unroll
     for (int i = 0; i < ND; i++) nls[i] = xp[i];
@@ -160,7 +195,7 @@ __device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, con
       W[q] = R(0);
       if (q < nw) W[q] = interp_dev<R, ND + 1, CE_MAX_NL_INSTRUCTIONS, false>(P->nl_ops[q], P->nl_n[q], nls, R(0));
     }
-    R ph[ND];
+    R ph[ND], pl[ND];
 #pragma unroll
     for (int i = 0; i < ND; i++) {
       R base = dbu[i] * u;
@@ -171,9 +206,18 @@ __device__ __forceinline__ void k_sim_body(const ProblemDev* __restrict__ P, con
 #pragma unroll
       for (int k = 0; k < ND; k++) acc = O::fma(dai[i * ND + k], xp[k], acc);
       ph[i] = acc;
+      R lo = R(0);
+      if (HL) {
+#pragma unroll
+        for (int k = 0; k < ND; k++) lo = O::fma(dai[i * ND + k], xpl[k], lo);
+      }
+      pl[i] = lo;
     }
 #pragma unroll
-    for (int i = 0; i < ND; i++) xp[i] = xp[i] + ph[i];
+    for (int i = 0; i < ND; i++) {
+      if constexpr (HL) hl_add(xp[i], xpl[i], ph[i], pl[i]);
+      else xp[i] = xp[i] + ph[i];
+    }
   }
   if (unstable) return;
   const R yss = ysum / (R)(T - s0);
