@@ -114,7 +114,7 @@ struct ce_batch {
   double* info64 = nullptr;
   unsigned long long* bmin = nullptr;
   int32_t* sfail = nullptr;
-  int32_t* counts = nullptr;   // count per problem
+  int32_t* counts = nullptr;   // 2 J: list length per problem, then nominal-list length per problem
   int32_t* rcounts = nullptr;  // refinement count per problem
   int32_t* fcounts = nullptr;  // FP64 sweep list length per problem
   std::vector<Launch> launches;
@@ -257,10 +257,10 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     return s;
   };
   const Seg s_info = seg(io * sizeof(double));
-  const Seg s_bmin = seg(no * 8), s_sfail = seg(no * 4), s_counts = seg(J * 4), s_rcounts = seg(J * 4);
+  const Seg s_bmin = seg(no * 8), s_sfail = seg(no * 4), s_counts = seg(2 * J * 4), s_rcounts = seg(J * 4);
   const Seg s_fcounts = seg(J * 4);
   struct PS {
-    Seg flags, uniq, mre, mim, oa, y, mag, list, mr32, refined, rlist, flist, atbp;
+    Seg flags, uniq, mre, mim, oa, y, mag, list, nlist, mr32, refined, rlist, flist, atbp;
     int parts, chunk;
   };
   std::vector<PS> ps(J);
@@ -276,6 +276,7 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     s.y = seg(n * (size_t)p.fc * (mixed ? 4 : 8));
     s.mag = seg(n * (size_t)p.F * (mixed ? 3 : 2) * 8);
     s.list = seg(n * 4);
+    s.nlist = seg(n * 4);
     if (mixed) {
       s.mr32 = seg(n * (size_t)p.F * 4 * 4);
       s.refined = seg(((n * (size_t)p.F + 3) / 4) * 4);
@@ -318,6 +319,8 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     B.sweep_fail = b->sfail + p.noff;
     B.list = (int32_t*)(W + s.list.off);
     B.count = b->counts + j;
+    B.nlist = (int32_t*)(W + s.nlist.off);
+    B.ncount = b->counts + J + j;
     if (mixed) {
       B.mr32 = (float*)(W + s.mr32.off);
       B.refined = (int8_t*)(W + s.refined.off);
@@ -486,7 +489,7 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   batch_mark(b, "start", st);
   launch_fill_u64(b->bmin, 0x7ff0000000000000ull, b->total_n, st);
   launch_fill_i32(b->sfail, 0, b->total_n, st);
-  launch_fill_i32(b->counts, 0, J, st);
+  launch_fill_i32(b->counts, 0, 2 * J, st);
   if (mixed) {
     launch_fill_i32(b->rcounts, 0, J, st);
     launch_fill_i32(b->fcounts, 0, J, st);

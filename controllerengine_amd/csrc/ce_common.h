@@ -44,7 +44,12 @@ struct ProblemDev {
   int32_t dc_n, u_n, nl_n[MAXW];
   int32_t fit_start_id, steady_start_id, fit_count, ref_step_id;
   int32_t info_w, info_scoring, info_freqresp, info_allmargin, info_modal, cuda_w;
-  int32_t serial, pad0;
+  int32_t serial;
+  // 1 when the margins stage's nominal closed loop is D LIN_A D, D = diag(I_ns, -I_ns), entry for
+  // entry (serial mode with nd = ns, or batch mode with the observer model equal to the plant and
+  // STATESPACE_D = 0): then it has the same eigenvalues, bit for bit, as LIN_A unless the LIN_A
+  // solve reports a sign-flip event (eig_grp)
+  int32_t nom_equiv;
   // scalars (double, promoted from the config's float values)
   double ts, fit_t0, fit_t1, unstable_err, eig_fail_err, ref_amp;
   double damping, bandwidth, overshoot, sse, target_gm, target_pm, target_b;

@@ -19,6 +19,14 @@
 // operations are seen by all of its lanes in issue order, so the fence needs no wait instruction
 // (the AMDGPU memory model emits none at wavefront scope).  Kernels using it run one wave per
 // workgroup.
+//
+// Sign-flip events.  For D = diag(+-1), every step of the solver maps the iterates of D A D onto D
+// times those of A times D, operation for operation (negation commutes with rounding), with two
+// exceptions: a Householder vector whose leading entry is zero while its tail is not (the sign of
+// beta then does not follow the vector), and the Francis start test |lhs| < eps rhs, whose rhs is
+// signed (it carries the sign of v0) and changes sign where D does.  With sign_block = k >= 0
+// (D = diag(I_k, -I)), *event is set when either happens; when it is not, the solve of D A D would
+// return the same eigenvalues (and the same status) bit for bit.
 #pragma once
 
 #include <float.h>
@@ -33,10 +41,11 @@ struct GHh {
   double e0, e1, tau, beta;
 };
 
-__device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int len) {
+__device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int len, int* event) {
   GHh h;
   double tail = cedm::fma_(v1, v1, 0.0);
   if (len == 3) tail = cedm::fma_(v2, v2, tail);
+  if (event && c0 == 0.0 && tail > DBL_MIN) *event = 1;
   if (tail <= DBL_MIN) {
     h.tau = 0.0;
     h.beta = c0;
@@ -60,7 +69,8 @@ __device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int 
 // within 40 n iterations, 2 = non-finite eigenvalue.
 #define CE_LANES(j) _Pragma("unroll") for (int q_ = 0, j = l; q_ < R; q_++, j += G)
 template <int G, int R = 1>
-__device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l) {
+__device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l, int sign_block = -1,
+                                       int* event = nullptr) {
   constexpr int S = G * R + 1;
   double scale = 0.0;
   for (int i = 0; i < n; i++)
@@ -84,6 +94,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
     double tail = 0.0;
     for (int k = 1; k < rem; k++) tail = cedm::fma_(T[(i + 1 + k) * S + i], T[(i + 1 + k) * S + i], tail);
     const double c0 = T[(i + 1) * S + i];
+    if (event && c0 == 0.0 && tail > DBL_MIN) *event = 1;
     double tau, beta, den = 1.0;
     const bool flat = tail <= DBL_MIN;
     if (flat) {
@@ -259,6 +270,8 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           const double lhs = T[im * S + im - 1] * (cedm::fabs_(v1) + cedm::fabs_(v2));
           const double rhs =
               v0 * (cedm::fabs_(T[(im - 1) * S + im - 1]) + cedm::fabs_(tmm) + cedm::fabs_(T[(im + 1) * S + im + 1]));
+          if (event && im == sign_block - 1 && (cedm::fabs_(lhs) < eps * rhs) != (cedm::fabs_(lhs) < eps * -rhs))
+            *event = 1;
           if (cedm::fabs_(lhs) < eps * rhs) break;
         }
         for (int k = im; k <= iu - 2; k++) {
@@ -266,7 +279,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           double w0, w1, w2;
           if (first) { w0 = v0; w1 = v1; w2 = v2; }
           else { w0 = T[k * S + k - 1]; w1 = T[(k + 1) * S + k - 1]; w2 = T[(k + 2) * S + k - 1]; }
-          const GHh h = grp_make_hh(w0, w1, w2, 3);
+          const GHh h = grp_make_hh(w0, w1, w2, 3, event);
           if (h.beta != 0.0) {
             grp_sync();
             if (l == 0) {
@@ -299,7 +312,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           }
         }
         {
-          const GHh h = grp_make_hh(T[(iu - 1) * S + iu - 2], T[iu * S + iu - 2], 0.0, 2);
+          const GHh h = grp_make_hh(T[(iu - 1) * S + iu - 2], T[iu * S + iu - 2], 0.0, 2, event);
           if (h.beta != 0.0) {
             grp_sync();
             if (l == 0) T[(iu - 1) * S + iu - 2] = h.beta;

@@ -64,7 +64,7 @@ struct ce_engine {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count;
+  Buf info64, flags, uniq, mode_re, mode_im, open_a, ybuf, ybuf32, mag_raw, bmin, sfail, list, count, nlist;
   Buf mr32, refined, rlist, rcount, flist, fcount, mres, atbp;
   // per-kernel timing (ce_engine_set_profiling)
   int profiling = 0;
@@ -203,7 +203,9 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   B.bmin = (unsigned long long*)ensure(E->bmin, nn * sizeof(unsigned long long));
   B.sweep_fail = (int32_t*)ensure(E->sfail, nn * sizeof(int32_t));
   B.list = (int32_t*)ensure(E->list, nn * sizeof(int32_t));
-  B.count = (int32_t*)ensure(E->count, sizeof(int32_t));
+  B.nlist = (int32_t*)ensure(E->nlist, nn * sizeof(int32_t));
+  B.count = (int32_t*)ensure(E->count, 2 * sizeof(int32_t));
+  B.ncount = B.count + 1;
   if (mixed) {
     B.mr32 = (float*)ensure(E->mr32, nn * (size_t)F * 4 * sizeof(float));
     B.refined = (int8_t*)ensure(E->refined, ((nn * (size_t)F + 3) / 4) * 4);
@@ -282,7 +284,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // early as possible
   launch_fill_u64(B.bmin, 0x7ff0000000000000ull, nn, st);  // +inf
   launch_fill_i32(B.sweep_fail, 0, nn, st);
-  launch_fill_i32(B.count, 0, 1, st);
+  launch_fill_i32(B.count, 0, 2, st);  // count and ncount
   if (mixed) {
     launch_fill_i32(B.rcount, 0, 1, st);
     launch_fill_i32(B.fcount, 0, 1, st);
@@ -366,7 +368,7 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (!E) return;
   hipDeviceSynchronize();
   for (auto* b : {&E->info64, &E->flags, &E->uniq, &E->mode_re, &E->mode_im, &E->open_a, &E->ybuf, &E->ybuf32,
-                  &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->mr32, &E->refined, &E->rlist,
+                  &E->mag_raw, &E->bmin, &E->sfail, &E->list, &E->count, &E->nlist, &E->mr32, &E->refined, &E->rlist,
                   &E->rcount, &E->flist, &E->fcount, &E->mres, &E->atbp, &E->gate})
     if (b->p) hipFree(b->p);
   if (E->d_prob) hipFree(E->d_prob);

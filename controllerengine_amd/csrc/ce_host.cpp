@@ -443,6 +443,16 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
       P.fnl_aval[q][r] = (float)P.nl_ops[q][r].aval;
       P.fnl_bval[q][r] = (float)P.nl_ops[q][r].bval;
     }
+  // nominal loop [[A1 - D L K', -L Pc'], [Pb K', Pa]] (src/allmargin.cu:63-71) against LIN_A
+  // [[A1, L Dc'], [-Dbu K', Da]] (src/eigensetup.cu:16-56): the sign flip of the plant half maps one
+  // onto the other when the margins plant is the simulated plant and D = 0
+  {
+    auto same = [](const float* a, const float* b, int k) { return std::memcmp(a, b, sizeof(float) * (size_t)k) == 0; };
+    bool eq = ns == nd;
+    if (eq && !c.serial)
+      eq = same(pl.ss_a, pl.d_a, ns * ns) && same(pl.ss_b, pl.d_bu, ns) && same(pl.ss_c, pl.d_c, ns) && pl.ss_d == 0.0f;
+    P.nom_equiv = eq ? 1 : 0;
+  }
   // margins plant: reference src/allmargin.cu:11-15
   if (c.serial) {
     for (int i = 0; i < ns * ns; i++) P.pa[i] = P.da[i];

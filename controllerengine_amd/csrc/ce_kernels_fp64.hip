@@ -34,22 +34,29 @@ __global__ __launch_bounds__(256) void k_info_final(const double* __restrict__ i
   if (i < count) out[i] = (float)in[i];
 }
 
-// Compaction: the candidates whose trajectory is needed (linearly stable ones, or all of them when
-// the trajectory dump is requested) get consecutive slots; one atomic per wave.
+// Compaction, one atomic per wave and list: the candidates whose trajectory is needed (linearly
+// stable ones, or all of them when the trajectory dump is requested) get consecutive slots of
+// list; those whose nominal loop needs its own eigensolve (not flagged unstable, skipped or
+// decided by the LIN_A solve) get consecutive slots of nlist.
+__device__ __forceinline__ void compact_append(bool pred, int c, int32_t* count, int32_t* list) {
+  const unsigned long long ballot = __ballot(pred);
+  const int cnt = __popcll(ballot);
+  int base = 0;
+  if (threadIdx.x == __ffsll((long long)ballot) - 1 && cnt > 0) base = atomicAdd(count, cnt);
+  base = __shfl(base, __ffsll((long long)ballot) - 1);
+  if (pred) {
+    const unsigned long long below = ballot & ((1ull << threadIdx.x) - 1ull);
+    list[base + __popcll(below)] = c;
+  }
+}
+
 __device__ __forceinline__ void k_compact_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx,
                                                int by, int bz) {
   const bool all = B.d_y != nullptr;
   const int c = bx * 64 + threadIdx.x;
-  const bool pred = c < B.n && (all || (B.flags[c] & F_UNSTABLE) == 0);
-  const unsigned long long ballot = __ballot(pred);
-  const int cnt = __popcll(ballot);
-  int base = 0;
-  if (threadIdx.x == __ffsll((long long)ballot) - 1 && cnt > 0) base = atomicAdd(B.count, cnt);
-  base = __shfl(base, __ffsll((long long)ballot) - 1);
-  if (pred) {
-    const unsigned long long below = ballot & ((1ull << threadIdx.x) - 1ull);
-    B.list[base + __popcll(below)] = c;
-  }
+  const int f = c < B.n ? B.flags[c] : F_UNSTABLE;
+  compact_append(c < B.n && (all || (f & F_UNSTABLE) == 0), c, B.count, B.list);
+  compact_append(c < B.n && (f & (F_UNSTABLE | F_NOMINAL_SKIPPED | F_NOMINAL_DONE)) == 0, c, B.ncount, B.nlist);
 }
 
 __global__ __launch_bounds__(64) void k_compact(const ProblemDev* __restrict__ P, StageBufs B) {
@@ -119,7 +126,17 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
     }
   }
   grp_sync();
-  const int rc = eig_grp<G, R>(T, V, WR, WI, nt, l);
+  int ev = 0;
+  const int rc = eig_grp<G, R>(T, V, WR, WI, nt, l, P->nom_equiv ? ns : -1, &ev);
+  // When the nominal loop of the margins stage is D LIN_A D entry for entry and this solve had no
+  // sign-flip event, the nominal eigensolve would return these eigenvalues and this status: its
+  // check (src/allmargin.cu:72-79: unstable when the solve fails or some |lambda| >= 1) is decided
+  // here and k_eig_nom skips the candidate.
+  const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
+  bool nom_out = false;
+  CE_ROWS(r) nom_out = nom_out || (rc == 0 && r < nt && cedm::hypot(WR[r], WI[r]) >= 1.0);
+  const bool nom_done = P->nom_equiv != 0 && ev == 0;
+  const bool nom_bad = rc != 0 || (__ballot(nom_out) & gmask) != 0ull;
   if (rc != 0) CE_ROWS(r) if (r < nt) { WR[r] = cedm::qnan(); WI[r] = cedm::qnan(); }
   grp_sync();
   double re[R], im[R];
@@ -143,7 +160,6 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
   }
   grp_sync();
   // modes in eigenvalue order: round q_ covers indices [q_ G, q_ G + G) in lane order
-  const unsigned long long gmask = ((1ull << G) - 1ull) << (gi * G);
   int ubase = 0;
   CE_ROWS(r) {
     bool keep = false;
@@ -172,6 +188,7 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
       flags |= F_UNSTABLE;
     }
     if ((flags & F_UNSTABLE) || sc[0] >= P->unstable_err) flags |= F_NOMINAL_SKIPPED;
+    else if (nom_done) flags |= F_NOMINAL_DONE | (nom_bad ? F_NOMINAL_UNSTABLE : 0);
     B.uniq[c] = ubase;
     B.flags[c] = flags;
   }
@@ -192,18 +209,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 
 // nominal closed loop [[A1 - D L K^T, -L Pc^T], [Pb K^T, Pa]] (src/allmargin.cu:63-79), for the
-// candidates of the compacted list that are not already flagged
+// candidates of the nominal list (k_compact): linearly stable, not rejected, not decided by the
+// LIN_A solve
 template <int G, int R>
 __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int N = G * R, S = N + 1, CPW = 64 / G;
   __shared__ double smat[CPW][N * S];
   __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
-  // candidates from the compacted list (after k_compact): the groups of a wave are all candidates
-  // that reach the simulation, most of which need the nominal check
+  // candidates from the nominal list (after k_compact), so that the groups of a wave all solve
   const int idx = bx * CPW + gi;
-  if (idx >= *B.count) return;
-  const int c = B.list[idx];
+  if (idx >= *B.ncount) return;
+  const int c = B.nlist[idx];
   const int flags = B.flags[c];
   if (flags & F_NOMINAL_SKIPPED) return;
   double* T = smat[gi];

@@ -11,8 +11,10 @@ enum : int32_t {
   F_UNSTABLE = 1,          // some closed-loop eigenvalue has |lambda| > 1 (reference src/modalsetup.cu:26)
   F_NOMINAL_UNSTABLE = 2,  // nominal loop of the margins stage unstable (reference src/allmargin.cu:61-87)
   F_NOMINAL_SKIPPED = 4,   // nominal check not run (candidate already at or above unstable_err)
-  F_SWEEP_FP64 = 8         // mixed sweep: every point recomputed in FP64 (more than RMAX points
+  F_SWEEP_FP64 = 8,        // mixed sweep: every point recomputed in FP64 (more than RMAX points
                            // wanting refinement)
+  F_NOMINAL_DONE = 16      // nominal check decided from the LIN_A solve (ProblemDev::nom_equiv and
+                           // no sign-flip event): the nominal eigensolve is not run
 };
 
 constexpr int RMAX = 64;   // FP64 refinement points per candidate in the mixed sweep
@@ -40,6 +42,8 @@ struct StageBufs {
   int32_t* sweep_fail;          // n
   int32_t* list;                // n: candidates that reach the simulation (compacted)
   int32_t* count;               // 1: entries in list
+  int32_t* nlist;               // n: candidates whose nominal loop needs its own eigensolve
+  int32_t* ncount;              // 1: entries in nlist
   // mixed sweep
   float* mr32;                  // n x F x 4: located |loop|, arg(loop), b(w), error estimate (float)
   int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite

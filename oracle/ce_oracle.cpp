@@ -179,11 +179,13 @@ struct HH {  // Householder vector of a 2- or 3-element segment
   double tau, beta;
 };
 
-// makeHouseholder: v = [c0; tail]
-void make_householder(const double* v, int len, HH& h) {
+// makeHouseholder: v = [c0; tail].  *event (when not NULL) is set when c0 is zero and the tail is
+// not: the one case in which the reflector of a sign-flipped vector is not the sign-flipped reflector.
+void make_householder(const double* v, int len, HH& h, int* event) {
   double tail_sq = 0.0;
   for (int k = 1; k < len; k++) tail_sq = cedm::fma_(v[k], v[k], tail_sq);
   const double c0 = v[0];
+  if (event && c0 == 0.0 && tail_sq > DBL_MIN) *event = 1;
   if (tail_sq <= DBL_MIN) {
     h.tau = 0.0;
     h.beta = c0;
@@ -236,7 +238,11 @@ void hh_right(double* A, int n, int r0, int c0, int rows, int cols, const double
   }
 }
 
-int eigvals(double* A, int n, double* wr, double* wi) {
+// sign_block / event (test instrumentation, both optional): with D = diag(I_{sign_block}, -I), the
+// solve of D A D takes the same steps and returns the same bits as the solve of A unless *event is
+// set: a Householder vector with a zero leading entry and a nonzero tail, or a Francis start test
+// at row sign_block - 1 whose outcome depends on the sign of its right-hand side.
+int eigvals(double* A, int n, double* wr, double* wi, int sign_block = -1, int* event = nullptr) {
   // RealSchur::compute: scale by the largest |a_ij|
   double scale = 0.0;
   for (int i = 0; i < n * n; i++) scale = std::max(scale, cedm::fabs_(A[i]));
@@ -256,6 +262,7 @@ int eigvals(double* A, int n, double* wr, double* wi) {
     for (int k = 1; k < rem; k++) tail_sq = cedm::fma_(v[k], v[k], tail_sq);
     const double c0 = v[0];
     double tau, beta, ess[CE_MAX_TOTAL_DIM];
+    if (event && c0 == 0.0 && tail_sq > DBL_MIN) *event = 1;
     if (tail_sq <= DBL_MIN) {
       tau = 0.0;
       beta = c0;
@@ -399,6 +406,8 @@ int eigvals(double* A, int n, double* wr, double* wi) {
           const double lhs = T[im * n + im - 1] * (cedm::fabs_(v[1]) + cedm::fabs_(v[2]));
           const double rhs = v[0] * (cedm::fabs_(T[(im - 1) * n + im - 1]) + cedm::fabs_(tmm) +
                                      cedm::fabs_(T[(im + 1) * n + im + 1]));
+          if (event && im == sign_block - 1 && (cedm::fabs_(lhs) < eps * rhs) != (cedm::fabs_(lhs) < eps * -rhs))
+            *event = 1;
           if (cedm::fabs_(lhs) < eps * rhs) break;
         }
         // performFrancisQRStep
@@ -411,7 +420,7 @@ int eigvals(double* A, int n, double* wr, double* wi) {
             w[0] = T[k * n + k - 1]; w[1] = T[(k + 1) * n + k - 1]; w[2] = T[(k + 2) * n + k - 1];
           }
           HH h;
-          make_householder(w, 3, h);
+          make_householder(w, 3, h, event);
           if (h.beta != 0.0) {
             if (first && k > il) T[k * n + k - 1] = -T[k * n + k - 1];
             else if (!first) T[k * n + k - 1] = h.beta;
@@ -422,7 +431,7 @@ int eigvals(double* A, int n, double* wr, double* wi) {
         {
           double w[2] = {T[(iu - 1) * n + iu - 2], T[iu * n + iu - 2]};
           HH h;
-          make_householder(w, 2, h);
+          make_householder(w, 2, h, event);
           if (h.beta != 0.0) {
             T[(iu - 1) * n + iu - 2] = h.beta;
             hh_left(T, n, iu - 1, iu - 1, 2, n - iu + 1, h.ess, h.tau);
@@ -1061,6 +1070,13 @@ ORA_EXPORT double ora_interp(const int* iops, const double* vops, int count, con
 ORA_EXPORT int ora_eigvals(const double* a, int n, double* wr, double* wi) {
   std::vector<double> A(a, a + (size_t)n * n);
   return eigvals(A.data(), n, wr, wi);
+}
+
+// As ora_eigvals, with the sign-flip events of the solve (see eigvals) for D = diag(I_sign_block, -I).
+ORA_EXPORT int ora_eigvals_events(const double* a, int n, int sign_block, double* wr, double* wi, int* event) {
+  std::vector<double> A(a, a + (size_t)n * n);
+  *event = 0;
+  return eigvals(A.data(), n, wr, wi, sign_block, event);
 }
 
 ORA_EXPORT void ora_ldlt_solve(const double* m, int n, const double* b, double* x) {

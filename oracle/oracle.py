@@ -32,6 +32,7 @@ def lib():
         _lib.ora_interp.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double]
         _lib.ora_interp.restype = ctypes.c_double
         _lib.ora_eigvals.argtypes = [vp, ctypes.c_int, vp, vp]
+        _lib.ora_eigvals_events.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         _lib.ora_ldlt_solve.argtypes = [vp, ctypes.c_int, vp, vp]
         _lib.ora_ldlt_solve.restype = None
         _lib.ora_shifted_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, vp]
@@ -76,6 +77,17 @@ def eigvals(a):
     wi = np.zeros(n)
     st = lib().ora_eigvals(_p(a), n, _p(wr), _p(wi))
     return st, wr + 1j * wi
+
+
+def eigvals_events(a, sign_block):
+    """eigvals plus the flag of the solve's sign-flip events for D = diag(I_sign_block, -I)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    wr = np.zeros(n)
+    wi = np.zeros(n)
+    ev = ctypes.c_int(0)
+    st = lib().ora_eigvals_events(_p(a), n, int(sign_block), _p(wr), _p(wi), ctypes.byref(ev))
+    return st, wr + 1j * wi, ev.value
 
 
 def ldlt_solve(m, b):
