@@ -207,6 +207,20 @@ CE_EXPORT int ce_server_drain(ce_server* s, double timeout_s);
 CE_EXPORT int ce_server_stats(ce_server* s, double* out, int n_max);
 CE_EXPORT int ce_server_reset_stats(ce_server* s);
 
+/* ---------------------------------------------------------------------------------------------
+ * Ranking and selection of a candidate block (no counterpart in the reference: its caller ranks
+ * the rows of INFO_MX_OUT.bin by the score column against the requirements of README.md:22-35).
+ * d_info: n x info_mx_width float rows on the device, as ce_evaluate writes them for cfg.
+ * d_order (k entries, may be NULL when k = 0): the first k candidate indices by ascending score
+ * (INFO column info_scoring); ties by index, -0 and +0 tie, scores that are not a number last.
+ * d_pass (n bytes, optional): 1 when the candidate passes -- stable cell 1, score below
+ * unstable_err, margin error 0 (GM, PM and b at or above their targets), overshoot at most the
+ * overshoot spec and |yss / ref_amplitude - 1| at most steady_state_error -- else 0.
+ * d_npass (optional): the number of passing candidates.  Enqueued on `stream`; device memory for
+ * the sort is allocated and released on that stream. */
+CE_EXPORT int ce_rank(const ce_config* cfg, const float* d_info, int n, int k, int32_t* d_order, uint8_t* d_pass,
+                      int32_t* d_npass, void* stream, char* err, int errlen);
+
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
 CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);

@@ -115,3 +115,49 @@ def test_batch_every_order_equals_single(engine, prec):
             assert np.array_equal(o32.view(np.uint32), s32.view(np.uint32)), (p.ns, p.name)
     finally:
         b.close()
+
+
+def test_c4_serial_rounds_in_one_batch(engine):
+    """The batched Monte Carlo workload (BASELINE config 4 at reduced size): serial-mode chain-drive
+    plants of 2-8 states with perturbed plants every round (reference serial rounds,
+    src/main.cu:467-470, src/config.cu:186-192), all in one batch with graph replay and autotuned
+    grids; each round's plants are uploaded with ce_batch_update.  FP64 rows equal the oracle's
+    bit for bit every round; mixed rows keep every branch and meet the margin tolerances, and equal
+    the single-problem chain's mixed rows."""
+    from controllerengine_amd.batch import Batch
+    from controllerengine_amd.metrics import compare
+    w = workloads.C4(seed=13, n_plants=10, rounds=3, nmin=64, nmax=400, tmin=600, tmax=1600, F=96)
+    assert len({p.ns for p in w.plants}) >= 3
+    b = Batch(0)
+    try:
+        for p in w.plants:
+            b.add(p, rnd=0)
+        b.set_options(use_graph=True, autotune=True)
+        for r in range(w.rounds):
+            for j, p in enumerate(w.plants):
+                b.update(j, p, rnd=r)
+            print("c4 round", r, "fp64", flush=True)
+            outs = b.evaluate(PRECISION_FP64)
+            assert b.info()["used_graph"] or r == 0
+            ref = []
+            for p, (o32, o64) in zip(w.plants, outs):
+                _, r64, _ = oracle.evaluate(p, rnd=r)
+                ref.append(r64)
+                same = (o64 == r64) | (np.isnan(o64) & np.isnan(r64))
+                assert same.all(), (p.name, r, int((~same).sum()))
+            print("c4 round", r, "mixed", flush=True)
+            outs = b.evaluate(PRECISION_MIXED)
+            for p, (o32, o64), r64 in zip(w.plants, outs, ref):
+                d = p.derived()
+                am, sc = d.info_allmargin, d.info_scoring
+                assert np.array_equal(o64[:, am + 3], r64[:, am + 3]), p.name
+                assert np.array_equal(o64[:, sc] >= 1e6, r64[:, sc] >= 1e6), p.name
+                cmp = compare(p, r64, o64)
+                assert cmp["max_dgm_db"] <= 0.01 and cmp["max_dpm_deg"] <= 0.01 and cmp["max_db"] <= 1e-4, (p.name, cmp)
+                assert cmp["passfail_mismatch"] == 0, (p.name, cmp)
+            if r == w.rounds - 1:
+                for p, (o32, o64) in list(zip(w.plants, outs))[:4]:
+                    _, s64 = engine.evaluate(p, precision=PRECISION_MIXED, rnd=r)
+                    assert np.array_equal(o64.view(np.uint64), s64.view(np.uint64)), p.name
+    finally:
+        b.close()
