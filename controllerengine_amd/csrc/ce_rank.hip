@@ -19,8 +19,10 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "ce_host.h"
 
@@ -100,6 +102,32 @@ void rank_enqueue(const RankRule& R, const float* info, int n, int k, int32_t* o
   if (e != hipSuccess) throw std::runtime_error(std::string("ce_rank: ") + hipGetErrorString(e));
 }
 
+// ce_rank_noalloc: the sort's double buffers (keys, indices) and its temporary storage live in one
+// static device array of a fixed size, which each device holds for the life of the process.  Host
+// code never passes the array's name to a kernel or a copy: its device address on the current
+// device comes from hipGetSymbolAddress, looked up once per device.
+constexpr int RANK_NOALLOC_MAX_N = 1 << 20;
+constexpr size_t RANK_NOALLOC_KV = (size_t)2 * RANK_NOALLOC_MAX_N * sizeof(uint32_t);  // 8 MiB per buffer pair
+constexpr size_t RANK_NOALLOC_TEMP = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[2 * RANK_NOALLOC_KV + RANK_NOALLOC_TEMP];
+
+void* rank_scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<void*> ptrs;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("ce_rank_noalloc: no current device");
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize((size_t)dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("ce_rank_noalloc: scratch address lookup failed: ") + hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
+
 void rank_check_args(const ce_config* cfg, const float* info, int n, int k, int32_t* order) {
   if (!cfg || !info) throw std::runtime_error("ce_rank: null argument");
   if (n <= 0) throw std::runtime_error("ce_rank: need n > 0");
@@ -134,6 +162,28 @@ extern "C" CE_EXPORT int ce_rank(const ce_config* cfg, const float* d_info, int 
       throw;
     }
     if (hipFreeAsync(buf, st) != hipSuccess) throw std::runtime_error("ce_rank: release failed");
+    return 0;
+  } catch (const std::exception& e) {
+    rank_put_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+extern "C" CE_EXPORT int ce_rank_noalloc(const ce_config* cfg, const float* d_info, int n, int k, int32_t* d_order,
+                                         uint8_t* d_pass, int32_t* d_npass, void* stream, char* err, int errlen) {
+  try {
+    rank_check_args(cfg, d_info, n, k, d_order);
+    if (n > RANK_NOALLOC_MAX_N)
+      throw std::runtime_error("ce_rank_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                               std::to_string(RANK_NOALLOC_MAX_N) + " candidates (use ce_rank)");
+    const RankRule R = rank_rule(*cfg);
+    const size_t tb = rank_temp_bytes(n);
+    if (tb > RANK_NOALLOC_TEMP)
+      throw std::runtime_error("ce_rank_noalloc: the sort needs " + std::to_string(tb) + " bytes of temporary storage, "
+                               "more than the " + std::to_string(RANK_NOALLOC_TEMP) + " reserved");
+    char* base = (char*)rank_scratch_ptr();
+    rank_enqueue(R, d_info, n, k, d_order, d_pass, d_npass, (uint32_t*)base, (int32_t*)(base + RANK_NOALLOC_KV),
+                 base + 2 * RANK_NOALLOC_KV, tb, (hipStream_t)stream);
     return 0;
   } catch (const std::exception& e) {
     rank_put_err(err, errlen, e.what());

@@ -221,6 +221,17 @@ CE_EXPORT int ce_server_reset_stats(ce_server* s);
 CE_EXPORT int ce_rank(const ce_config* cfg, const float* d_info, int n, int k, int32_t* d_order, uint8_t* d_pass,
                       int32_t* d_npass, void* stream, char* err, int errlen);
 
+/* ce_rank without device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments, results and errors as ce_rank, for n <= CE_RANK_NOALLOC_MAX_N;
+ * a larger n is rejected with an error.  The sort's buffers are a static device array of 20 MiB that
+ * every GPU the library runs on holds until the process ends.  All calls on one GPU share that
+ * GPU's array: calls that overlap on one GPU (other streams or threads) race, and the caller must
+ * serialise them. */
+enum { CE_RANK_NOALLOC_MAX_N = 1 << 20 };
+CE_EXPORT int ce_rank_noalloc(const ce_config* cfg, const float* d_info, int n, int k, int32_t* d_order,
+                              uint8_t* d_pass, int32_t* d_npass, void* stream, char* err, int errlen);
+
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
  * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
 CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);
