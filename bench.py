@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3     # published (packed FP32); this build issues non-pa
 HBM_PEAK_GBS = 8000.0
 METRIC = "candidate evals/s, mixed vs FP64 plain at matched margins; p50/p99 job latency"
 
-F32_KERNELS = ("ce_sim_f32", "ce_sim_f32_g", "k_sim_f32", "k_sweep_mark32")
+F32_KERNELS = ("ce_sim_f32", "ce_sim_f32_g", "ce_sims_f32", "k_sim_f32", "k_sweep_mark32")
 WORKLOADS = {"c1": "mass-spring-damper 2-state", "c2": "two-mass drive 4-state, 2 NL",
              "c3": "random stable 8-state, 4 NL, Kalman observer"}
 
@@ -63,6 +63,7 @@ def kernel_flops(problem, info64, d, mixed):
     swept = int((info64[:, d.info_allmargin + 3] == 1).sum())
     return {
         "ce_sim_f32": sim, "ce_sim_f32_g": sim, "ce_sim_f64": sim, "ce_sim_f64_g": sim, "k_sim_f32": sim,
+        "ce_sims_f32": sim, "ce_sims_f64": sim,
         "k_sim_fp64": sim,
         "k_eig_lin": float(10 * nt ** 3 + 4 * ns * ns + 2 * ns * nd + 2 * ns) * n,
         "k_eig_nom": float(10 * (2 * ns) ** 3) * n_sim,

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -84,6 +85,12 @@ struct ce_engine {
   // compilation failed); reset when a different problem is uploaded
   void* sim_fn[2][2] = {};
   bool sim_fn_known[2][2] = {};
+  bool sim_fn_struct[2][2] = {};  // the structural form (constants from the descriptor)
+  // structures (sim_structure_key, gated or not) whose literal kernel this engine has compiled: a
+  // second problem of the same structure means the constants change between calls (serial rounds,
+  // a search over plants), and it gets the structural kernel, compiled once per structure
+  std::set<std::string> literal_structs;
+  long long jit_evictions_seen = 0;
   int last_sim_jit = 0;
   int last_mixed = 0;
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
@@ -256,6 +263,10 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   bool gated = false;
   if (jit_enabled()) {
     if (!E->jit) E->jit = jit_cache_create();
+    if (jit_evictions(E->jit) != E->jit_evictions_seen) {  // a cached kernel may have been unloaded
+      for (auto& row : E->sim_fn_known) row[0] = row[1] = false;
+      E->jit_evictions_seen = jit_evictions(E->jit);
+    }
     // resolved once per uploaded problem: generating the source to look it up takes tens of
     // microseconds of host time, during which the device would wait for its next kernel
     auto lookup = [&](bool g) -> void* {
@@ -263,8 +274,18 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
       if (!E->sim_fn_known[mi][gi]) {
         std::string jerr;
         void* f = nullptr;
-        if (!jit_get(E->jit, gen_sim_source(H, mixed, g), &f, jerr)) f = nullptr;
-        E->sim_fn[mi][gi] = f;
+        const std::string lit = gen_sim_source(H, mixed, g);
+        const std::string skey = sim_structure_key(H, mixed) + (g ? "|gated" : "");
+        const bool structural = !jit_cached(E->jit, lit) && E->literal_structs.count(skey) != 0;
+        bool ok;
+        if (structural) {
+          ok = jit_get(E->jit, gen_sim_struct_source(H, mixed, g), &f, jerr, false);
+        } else {
+          ok = jit_get(E->jit, lit, &f, jerr, true);
+          if (ok) E->literal_structs.insert(skey);
+        }
+        E->sim_fn[mi][gi] = ok ? f : nullptr;
+        E->sim_fn_struct[mi][gi] = structural;
         E->sim_fn_known[mi][gi] = true;
       }
       return E->sim_fn[mi][gi];
@@ -312,11 +333,18 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;
   E->last_sim_jit = 0;
   if (fn) {
-    const bool ok = gated ? jit_launch_sim_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info,
-                                                 B.d_y, n, (int*)E->gate.p, E->gate_gen)
-                          : jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n);
+    const bool st_form = E->sim_fn_struct[mixed ? 1 : 0][gated ? 1 : 0];
+    bool ok;
+    if (st_form)
+      ok = gated ? jit_launch_sim_struct_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
+                                               n, (int*)E->gate.p, E->gate_gen, P)
+                 : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P);
+    else
+      ok = gated ? jit_launch_sim_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
+                                        (int*)E->gate.p, E->gate_gen)
+                 : jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n);
     if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");
-    kmark(mixed ? "ce_sim_f32" : "ce_sim_f64", st);
+    kmark(st_form ? (mixed ? "ce_sims_f32" : "ce_sims_f64") : (mixed ? "ce_sim_f32" : "ce_sim_f64"), st);
     E->last_sim_jit = 1;
   } else {
     const bool ok = mixed ? launch_sim_mixed(ns, H.nd, P, B, st) : launch_sim_fp64(ns, H.nd, P, B, st);
@@ -417,6 +445,9 @@ extern "C" CE_EXPORT int ce_engine_stage_times(ce_engine* E, double* ms, int n_m
 
 // 1 when the last evaluation's simulation stage ran the runtime-specialised kernel, else 0.
 extern "C" CE_EXPORT int ce_engine_last_sim_jit(ce_engine* E) { return E ? E->last_sim_jit : 0; }
+
+// hipRTC compilations this engine has done.
+extern "C" CE_EXPORT long long ce_engine_jit_compiles(ce_engine* E) { return E ? jit_compiles(E->jit) : 0; }
 
 // Mixed sweep counts of the last evaluation: points recomputed in FP64 and candidates whose whole
 // sweep was.  Waits for the device.

@@ -16,8 +16,10 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <list>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -366,30 +368,57 @@ static const char* kSimArgs =
 //  * stores gate_gen to *gate when its last workgroup starts: by then the workgroups before it have
 //    been placed, and a bounded waiter on the second stream lets the kernels there start, on the
 //    SIMDs the simulation leaves free.
-std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated) {
+// Body of a structural kernel: constants loaded from the descriptor, hoisted out of the time loop
+// when they fit a modest register budget (kHoistMaxWords 32-bit words), else read every step.
+static void gen_struct_body(std::ostringstream& o, const ProblemDev& P, bool mixed) {
+  constexpr int kHoistMaxWords = 64;
+  Gen gh;
+  gh.f32 = mixed;
+  int words = 0;
+  gen_body(gh, P, mixed, true, true, &words);
+  if (words <= kHoistMaxWords) {
+    o << gh.o.str();
+  } else {
+    Gen gr;
+    gr.f32 = mixed;
+    gen_body(gr, P, mixed, true, false);
+    o << gr.o.str();
+  }
+}
+
+// Single-problem kernel entry: the literal form (every constant in the source) or the structural
+// form (constants read from the problem descriptor passed as the last argument).
+static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool structural) {
   Gen g;
   g.f32 = mixed;
   std::ostringstream& o = g.o;
-  const char* name = gated ? (mixed ? "ce_sim_f32_g" : "ce_sim_f64_g") : (mixed ? "ce_sim_f32" : "ce_sim_f64");
+  const std::string name = std::string(structural ? "ce_sims_" : "ce_sim_") + (mixed ? "f32" : "f64") + (gated ? "_g" : "");
   o << "// kernel: " << name << "\n";
-  o << "// generated: closed-loop simulation, ns=" << P.ns << " nd=" << P.nd << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
+  o << "// generated: closed-loop simulation" << (structural ? ", constants from the problem descriptor" : "")
+    << ", ns=" << P.ns << " nd=" << P.nd << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
   o << kSimArgs;
   o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
-  gen_body(g, P, mixed, false);
+  if (structural) gen_struct_body(o, P, mixed);
+  else gen_body(g, P, mixed, false);
   o << "}\n";
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
        "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
-       "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "") << ") {\n"
+       "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
+    << (structural ? ", const char* __restrict__ probp" : "") << ") {\n"
        "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
   if (gated)
     o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
          "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
          "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
   o << "  __builtin_amdgcn_s_setprio(3);\n"
-       "  sim_one(A, nullptr, blockIdx.x);\n}\n";
+    << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
   return o.str();
 }
+
+std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated) { return gen_single(P, mixed, gated, false); }
+
+std::string gen_sim_struct_source(const ProblemDev& P, bool mixed, bool gated) { return gen_single(P, mixed, gated, true); }
 
 std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
   Gen g;
@@ -401,22 +430,7 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
     << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
   o << kSimArgs;
   o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
-  {
-    // constants hoisted out of the time loop when they fit in a modest register budget
-    constexpr int kHoistMaxWords = 64;
-    Gen gh;
-    gh.f32 = mixed;
-    int words = 0;
-    gen_body(gh, P, mixed, true, true, &words);
-    if (words <= kHoistMaxWords) {
-      o << gh.o.str();
-    } else {
-      Gen gr;
-      gr.f32 = mixed;
-      gen_body(gr, P, mixed, true, false);
-      o << gr.o.str();
-    }
-  }
+  gen_struct_body(o, P, mixed);
   o << "}\n";
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
@@ -439,9 +453,15 @@ struct JitEntry {
   hipFunction_t fn = nullptr;
 };
 
+// Compiled modules by source text.  Entries added as evictable (the literal kernels, one per
+// distinct set of plant constants) are kept to the JIT_EVICTABLE_MAX most recently used; the rest
+// (structural kernels, one per instruction structure) stay until the cache is destroyed.
+constexpr size_t JIT_EVICTABLE_MAX = 32;
 struct JitCache {
   std::mutex mu;
   std::map<std::string, JitEntry> entries;
+  std::list<std::string> lru;  // evictable entries, most recently used first
+  long long compiles = 0, evictions = 0;
 };
 
 JitCache* jit_cache_create() { return new JitCache(); }
@@ -458,10 +478,29 @@ bool jit_enabled() {
   return !(v && std::string(v) == "0");
 }
 
-bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string& err) {
+bool jit_cached(JitCache* cache, const std::string& src) {
+  std::lock_guard<std::mutex> lk(cache->mu);
+  return cache->entries.count(src) != 0;
+}
+
+long long jit_compiles(JitCache* cache) {
+  if (!cache) return 0;
+  std::lock_guard<std::mutex> lk(cache->mu);
+  return cache->compiles;
+}
+
+long long jit_evictions(JitCache* cache) {
+  if (!cache) return 0;
+  std::lock_guard<std::mutex> lk(cache->mu);
+  return cache->evictions;
+}
+
+bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string& err, bool evictable) {
   std::lock_guard<std::mutex> lk(cache->mu);
   auto it = cache->entries.find(src);
   if (it != cache->entries.end()) {
+    auto li = std::find(cache->lru.begin(), cache->lru.end(), src);
+    if (li != cache->lru.end()) cache->lru.splice(cache->lru.begin(), cache->lru, li);
     *fn_out = (void*)it->second.fn;
     return true;
   }
@@ -501,6 +540,22 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
     return false;
   }
   cache->entries[src] = e;
+  cache->compiles++;
+  if (evictable) {
+    cache->lru.push_front(src);
+    if (cache->lru.size() > JIT_EVICTABLE_MAX) {
+      // the module may still have launches in flight: drain the device before unloading it
+      hipDeviceSynchronize();
+      const std::string old = cache->lru.back();
+      cache->lru.pop_back();
+      auto oi = cache->entries.find(old);
+      if (oi != cache->entries.end()) {
+        hipModuleUnload(oi->second.mod);
+        cache->entries.erase(oi);
+      }
+      cache->evictions++;
+    }
+  }
   *fn_out = (void*)e.fn;
   return true;
 }
@@ -514,6 +569,19 @@ bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, cons
 bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                           const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen) {
   void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gate, &gen};
+  return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
+}
+
+bool jit_launch_sim_struct(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &prob};
+  return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
+}
+
+bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                                 const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen,
+                                 const void* prob) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gate, &gen, &prob};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

@@ -17,7 +17,21 @@ bool jit_enabled();
 // constant written into the source.  gated: the form that holds whole SIMDs and signals its start
 // (jit_launch_sim_gated; grid at most the device's SIMD count).
 std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated = false);
-bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err);
+// Compiled kernel of src (compiled on first use).  evictable: the entry may be unloaded once more
+// recent evictable entries exceed the cache's capacity (after a device drain).
+bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err, bool evictable = false);
+bool jit_cached(JitCache* cache, const std::string& src);
+long long jit_compiles(JitCache* cache);   // hipRTC compilations done by this cache
+long long jit_evictions(JitCache* cache);  // entries unloaded
+// Structural single-problem form: constants and sizes read from the problem descriptor (the last
+// kernel argument), so one compiled kernel serves every problem of the same dimensions and
+// instruction structure (sim_structure_key), e.g. the rounds of a serial-mode run.
+std::string gen_sim_struct_source(const ProblemDev& P, bool mixed, bool gated = false);
+bool jit_launch_sim_struct(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob);
+bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
+                                 const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen,
+                                 const void* prob);
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n);
 // the gated form: its last workgroup stores gen to *gate when it starts

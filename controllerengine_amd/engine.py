@@ -89,6 +89,10 @@ class Engine:
         k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)
         return [buf[i] for i in range(k)]
 
+    def jit_compiles(self):
+        """hipRTC compilations of this engine (simulation kernels)."""
+        return int(_abi.lib().ce_engine_jit_compiles(self.h))
+
     def sweep_counts(self):
         """(FP64-refined points, candidates swept entirely in FP64) of the last mixed evaluate."""
         r, f = ctypes.c_int(0), ctypes.c_int(0)

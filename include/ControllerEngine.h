@@ -142,6 +142,13 @@ CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
 /* 1 when the last ce_evaluate ran the runtime-specialised (hipRTC) simulation kernel.  The engine
  * specialises the per-step code of a problem unless CE_JIT=0 is set in the environment. */
 CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
+/* hipRTC compilations this engine has done (no counterpart in the reference).  A problem is
+ * compiled with its constants written into the source; once the engine sees a second problem of
+ * the same dimensions and instruction structure (the rounds of a serial-mode run, a search over
+ * plants), it compiles that structure once more with the constants read from the device
+ * descriptor and uses that kernel for every later problem of the structure.  At most 32 kernels of
+ * the first kind are kept (least recently used first out). */
+CE_EXPORT long long ce_engine_jit_compiles(ce_engine* eng);
 /* Per-kernel timing (no counterpart in the reference).  With profiling on, ce_evaluate records an
  * event after every kernel it launches; ce_engine_kernel_times then returns, for the last call, the
  * number of kernels k (<= n_max), their durations in ms and their names, comma-separated, in
