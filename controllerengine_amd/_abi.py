@@ -73,7 +73,7 @@ EXPORTS = [
     "ce_detmath_device", "ce_batch_create", "ce_batch_destroy", "ce_batch_clear", "ce_batch_add", "ce_batch_update",
     "ce_batch_offsets", "ce_batch_set_options", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
     "ce_server_create", "ce_server_destroy", "ce_server_submit", "ce_server_poll", "ce_server_drain",
-    "ce_server_stats", "ce_server_reset_stats", "ce_rank", "ce_rank_noalloc",
+    "ce_server_stats", "ce_server_reset_stats", "ce_rank", "ce_rank_noalloc", "ce_simulate_kalman",
 ]
 
 _lib = None
@@ -117,6 +117,9 @@ def _declare(lib):
     lib.ce_detmath_device.restype = ctypes.c_int
     lib.ce_rank.argtypes = [P(CeConfig), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
     lib.ce_rank.restype = ctypes.c_int
+    lib.ce_simulate_kalman.argtypes = [vp, P(CeConfig), P(CePlant), ctypes.c_int, vp, ctypes.c_int, vp, vp, vp,
+                                       ctypes.c_char_p, ctypes.c_int]
+    lib.ce_simulate_kalman.restype = ctypes.c_int
     lib.ce_rank_noalloc.argtypes = [P(CeConfig), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, ctypes.c_char_p,
                                     ctypes.c_int]
     lib.ce_rank_noalloc.restype = ctypes.c_int

@@ -39,6 +39,8 @@ void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st);
 void launch_gate_wait(const int* gate, int gen, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
+bool launch_sim_kalman(int ns, int nd, bool mixed, const ProblemDev* P, const float* gains, int n, double* y64,
+                       float* y32, hipStream_t st);
 }  // namespace ce
 
 using namespace ce;
@@ -421,6 +423,37 @@ extern "C" CE_EXPORT int ce_evaluate(ce_engine* E, const ce_config* cfg, const c
     return 0;
   } catch (const std::exception& e) {
     put_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// Kalman-filter simulation (reference src/discrete_solves.cu:83-185): trajectories only.
+extern "C" CE_EXPORT int ce_simulate_kalman(ce_engine* E, const ce_config* cfg, const ce_plant* plant, int precision,
+                                            const float* d_gains, int n, double* d_y64, float* d_y32, void* stream,
+                                            char* err, int errlen) {
+  try {
+    if (!E || !cfg || !plant) throw std::runtime_error("ce_simulate_kalman: null argument");
+    if (precision != CE_PRECISION_FP64 && precision != CE_PRECISION_MIXED)
+      throw std::runtime_error("ce_simulate_kalman: unknown precision " + std::to_string(precision));
+    const bool mixed = precision == CE_PRECISION_MIXED;
+    if (n < 0) throw std::runtime_error("ce_simulate_kalman: negative candidate count");
+    if (n > 0 && (!d_gains || (mixed ? !d_y32 : !d_y64)))
+      throw std::runtime_error("ce_simulate_kalman: missing gains or output buffer for this precision");
+    HIPCHK(hipSetDevice(E->device));
+    std::vector<uint8_t> blob;
+    std::string e;
+    // the margins stage's plant gain at the crossover is not needed here
+    if (!build_problem(*cfg, *plant, blob, e) && e.rfind("allmargin:", 0) != 0) throw std::runtime_error(e);
+    if (n == 0) return 0;
+    const ProblemDev& H = *reinterpret_cast<const ProblemDev*>(blob.data());
+    const hipStream_t st = (hipStream_t)stream;
+    const ProblemDev* P = upload_problem(E, blob, st);
+    if (!launch_sim_kalman(H.ns, H.nd, mixed, P, d_gains, n, d_y64, d_y32, st))
+      throw std::runtime_error("no Kalman simulation kernel for these dimensions");
+    HIPCHK(hipGetLastError());
+    return 0;
+  } catch (const std::exception& ex) {
+    put_err(err, errlen, ex.what());
     return 1;
   }
 }

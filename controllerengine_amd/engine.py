@@ -84,6 +84,25 @@ class Engine:
         torch.cuda.synchronize(self.device)
         return i_out.cpu().numpy(), (i64.cpu().numpy() if want64 else None)
 
+    def simulate_kalman(self, problem, gains, precision=_abi.PRECISION_FP64, rnd=0):
+        """Kalman-filter simulation (ce_simulate_kalman) of gains rows [K, N, R, Q] (2 ns + 2
+        floats); returns the trajectories, n x time (double in FP64, float in mixed)."""
+        torch = self.torch
+        g = self.to_device(np.ascontiguousarray(gains, dtype=np.float32))
+        n = int(g.shape[0])
+        T = int(problem.config["time"])
+        mixed = precision == _abi.PRECISION_MIXED
+        y = torch.empty((n, T), dtype=torch.float32 if mixed else torch.float64, device=g.device)
+        err = _abi.errbuf()
+        rc = _abi.lib().ce_simulate_kalman(self.h, ctypes.byref(problem.cfg_struct()),
+                                           ctypes.byref(problem.plant_struct(rnd)), int(precision), _dev_ptr(g), n,
+                                           None if mixed else _dev_ptr(y), _dev_ptr(y) if mixed else None,
+                                           self.stream_handle(), err, len(err))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        torch.cuda.synchronize(self.device)
+        return y.cpu().numpy()
+
     def stage_times(self):
         buf = (ctypes.c_double * 8)()
         k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)

@@ -130,6 +130,19 @@ CE_EXPORT int ce_evaluate(ce_engine* eng, const ce_config* cfg, const ce_plant* 
                           const float* d_gains, const float* d_info_in, float* d_info_out, double* d_info64_out,
                           int n, void* stream, char* err, int errlen);
 
+/* Kalman-filter closed-loop simulation: the reference's discrete_solve<ControllerType::Kalman>
+ * (src/discrete_solves.cu:83-185), which the reference never launches.  Trajectories y (n x time)
+ * only: in FP64 into d_y64 (double), in mixed precision into d_y32 (float, compensated state).
+ * d_gains: n rows of [K (statespace_dim), N, R, Q (statespace_dim)] -- 2 statespace_dim + 2 floats;
+ * the reference sizes a gains row 2 statespace_dim + 1 (src/config.cu:51) and so reads the last Q
+ * entry from the next row.  Per step: predict the state and the covariance P = A P A' + diag(Q),
+ * gain KG = P C' / (C P C' + R), update xhat += KG (y - C xhat) and
+ * P = (I - KG C) P (I - KG C)' + R KG KG' (the product the reference's in-place update intends;
+ * its fast reciprocal is an IEEE division here).  Enqueued on `stream`. */
+CE_EXPORT int ce_simulate_kalman(ce_engine* eng, const ce_config* cfg, const ce_plant* plant, int precision,
+                                 const float* d_gains, int n, double* d_y64, float* d_y32, void* stream, char* err,
+                                 int errlen);
+
 /* Optional per-stage device outputs of the last ce_evaluate call (the reference's dump_diagnostics
  * files, src/main.cu:483-547).  Sizes per candidate: yplant time, lina nt*nt, eigen 2*nt,
  * ata (2nt)^2 (lower triangle, row stride 2nt), atb 2nt.  All float, candidate-major like the

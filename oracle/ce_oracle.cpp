@@ -6,6 +6,7 @@
 //   decode_instructions     src/main.cu:317-337 (+ NL y-slot rule src/main.cu:377)
 //   interpreter             src/main.cu:99-105, 260-291
 //   discrete_solve<Luen.>   src/discrete_solves.cu:12-79   (state in double, no hi/lo pairs)
+//   discrete_solve<Kalman>  src/discrete_solves.cu:83-185  (ora_simulate_kalman; gains row 2n+2)
 //   eigensetup              src/eigensetup.cu:1-57
 //   eigensolve              src/eigensolve.cu:1-19         (Eigen EigenSolver<double>, restated)
 //   modalsetup              src/modalsetup.cu:1-97
@@ -1029,6 +1030,106 @@ void run_candidate(const Problem& P, const float* gains, double* info, const Dia
   sc[0] = cedm::isfinite_(total) ? total : EF;
 }
 
+// discrete_solve<Kalman> (src/discrete_solves.cu:83-185), never launched by the reference, restated in
+// double: gains row [K (ns), N, R, Q (ns)] of 2 ns + 2 floats (the reference sizes the row 2 ns + 1,
+// src/config.cu:51, and reads one float past it).  Per step: y = DC(xp); u = U(N r - K xh); predict
+// xh += (A - I) xh + B u; P = A P A' + diag(Q); gain KG = P C' / (C P C' + R); update
+// xh += KG (y - C xh); P = (I - KG C) P (I - KG C)' + R KG KG'; then the plant step of the
+// Luenberger simulator.  The reference's covariance update is written in place through restrict
+// pointers (ABATf(IKHP, P, IKHP)); this is the product it intends.  Its fast reciprocal
+// (__fdividef) is an IEEE division here.  y_out: n x time.
+static void kalman_candidate(const Problem& P, const float* gains, double* y) {
+  const ce_config& c = P.cfg;
+  const int ns = P.ns, nd = P.nd, T = c.time;
+  double K[8], Q[8];
+  for (int i = 0; i < ns; i++) { K[i] = gains[i]; Q[i] = gains[ns + 2 + i]; }
+  const double N = gains[ns], R = gains[ns + 1];
+  double xh[8] = {0}, xp[8] = {0}, Pm[64] = {0};
+  double nls[CE_MAX_NL_DIM];
+  for (int t = 0; t < T; t++) {
+    const double yv = interp(P.dc_ops, c.dc_instructions, xp, false, 0.0);
+    y[t] = yv;
+    double kx = 0.0;
+    for (int k = 0; k < ns; k++) kx = cedm::fma_(K[k], xh[k], kx);
+    const double u_raw = cedm::fma_(N, P.ref[t], -kx);
+    const double u = interp(P.u_ops, c.u_instructions, xh, true, u_raw);
+    double dh[8];
+    for (int i = 0; i < ns; i++) {
+      double acc = P.sb[i] * u;
+      for (int k = 0; k < ns; k++) acc = cedm::fma_(P.sai[i * ns + k], xh[k], acc);
+      dh[i] = acc;
+    }
+    for (int i = 0; i < ns; i++) xh[i] = xh[i] + dh[i];
+    // P = A P A' + diag(Q)
+    double C0[64], Pn[64];
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) {
+        double a = 0.0;
+        for (int k = 0; k < ns; k++) a = cedm::fma_(P.sa[i * ns + k], Pm[k * ns + j], a);
+        C0[i * ns + j] = a;
+      }
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) {
+        double a = 0.0;
+        for (int k = 0; k < ns; k++) a = cedm::fma_(C0[i * ns + k], P.sa[j * ns + k], a);
+        Pn[i * ns + j] = a;
+      }
+    for (int i = 0; i < ns; i++) Pn[i * ns + i] = Pn[i * ns + i] + Q[i];
+    // KG = P C' / (C P C' + R)
+    double pht[8], cp[8];
+    for (int i = 0; i < ns; i++) {
+      double a = 0.0;
+      for (int k = 0; k < ns; k++) a = cedm::fma_(Pn[i * ns + k], P.sc[k], a);
+      pht[i] = a;
+    }
+    for (int j = 0; j < ns; j++) {
+      double a = 0.0;
+      for (int k = 0; k < ns; k++) a = cedm::fma_(P.sc[k], Pn[k * ns + j], a);
+      cp[j] = a;
+    }
+    double hphr = 0.0;
+    for (int k = 0; k < ns; k++) hphr = cedm::fma_(cp[k], P.sc[k], hphr);
+    hphr = 1.0 / (hphr + R);
+    double kg[8];
+    for (int i = 0; i < ns; i++) kg[i] = pht[i] * hphr;
+    // xh += KG (y - C xh)
+    double dz = -yv;
+    for (int k = 0; k < ns; k++) dz = cedm::fma_(P.sc[k], xh[k], dz);
+    dz = -dz;
+    for (int i = 0; i < ns; i++) xh[i] = xh[i] + kg[i] * dz;
+    // P = (I - KG C) P (I - KG C)' + R KG KG'
+    double ikh[64];
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) ikh[i * ns + j] = (i == j ? 1.0 : 0.0) - kg[i] * P.sc[j];
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) {
+        double a = 0.0;
+        for (int k = 0; k < ns; k++) a = cedm::fma_(ikh[i * ns + k], Pn[k * ns + j], a);
+        C0[i * ns + j] = a;
+      }
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) {
+        double a = 0.0;
+        for (int k = 0; k < ns; k++) a = cedm::fma_(C0[i * ns + k], ikh[j * ns + k], a);
+        Pm[i * ns + j] = a + (kg[i] * R) * kg[j];
+      }
+    // plant step, as the Luenberger simulator
+    for (int i = 0; i < nd; i++) nls[i] = xp[i];
+    nls[nd] = yv;
+    double W[4];
+    for (int q = 0; q < P.nw; q++) W[q] = interp(P.nl_ops[q], P.nl_count[q], nls, false, 0.0);
+    double ph[8];
+    for (int i = 0; i < nd; i++) {
+      double base = P.dbu[i] * u;
+      for (int q = 0; q < P.nw; q++) base = cedm::fma_(P.dbw[i * P.nw + q], W[q], base);
+      double acc = base;
+      for (int k = 0; k < nd; k++) acc = cedm::fma_(P.dai[i * nd + k], xp[k], acc);
+      ph[i] = acc;
+    }
+    for (int i = 0; i < nd; i++) xp[i] = xp[i] + ph[i];
+  }
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -1152,6 +1253,17 @@ ORA_EXPORT int ora_evaluate(const ce_config* cfg, const ce_plant* plant, const f
         for (int k = 0; k < iw; k++) info_out[(size_t)t * iw + k] = (float)row[k];
     }
   }
+  return 0;
+}
+
+ORA_EXPORT int ora_simulate_kalman(const ce_config* cfg, const ce_plant* plant, const float* gains, int n, double* y_out,
+                                   char* err, int errlen) {
+  Problem P;
+  std::string e;
+  if (!setup(*cfg, *plant, P, e)) { set_err(err, errlen, e); return 1; }
+  const int gw = 2 * P.ns + 2;
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int t = 0; t < n; t++) kalman_candidate(P, gains + (size_t)t * gw, y_out + (size_t)t * cfg->time);
   return 0;
 }
 

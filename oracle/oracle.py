@@ -39,6 +39,8 @@ def lib():
         _lib.ora_detmath.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int]
         _lib.ora_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
         _lib.ora_max_threads.restype = ctypes.c_int
+        _lib.ora_simulate_kalman.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_int]
+        _lib.ora_simulate_kalman.restype = ctypes.c_int
     return _lib
 
 
@@ -68,6 +70,20 @@ def evaluate(problem, rnd=0, gains=None, info=None, threads=0, diag=False):
     if rc != 0:
         raise ValueError(err.value.decode())
     return out32, out64, dg
+
+
+def simulate_kalman(problem, gains, rnd=0):
+    """Trajectories (n x time, double) of the Kalman simulator for gains rows [K, N, R, Q] (2 ns + 2)."""
+    cfg = problem.cfg_struct()
+    plant = problem.plant_struct(rnd)
+    g = np.ascontiguousarray(gains, dtype=np.float32)
+    n = g.shape[0]
+    y = np.zeros((n, cfg.time))
+    err = ctypes.create_string_buffer(512)
+    rc = lib().ora_simulate_kalman(ctypes.byref(cfg), ctypes.byref(plant), _p(g), n, _p(y), err, len(err))
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return y
 
 
 def eigvals(a):

@@ -16,7 +16,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "ce_oracle.cpp")
-PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_pipeline.py", "tests/test_oracle_linalg.py"]
+PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_pipeline.py", "tests/test_oracle_linalg.py",
+             "tests/test_oracle_kalman.py"]
 
 # (cell, text in the oracle, the same text with the mistake)
 MUTANTS = [
@@ -69,7 +70,12 @@ MUTANTS = [
      "const float inc = (cedm::logf_(P.d.freq_max) - cedm::logf_(c.freq_min)) / (float)F;"),
     ("grid in double", "P.z[w] = Cx{(double)co, (double)s};",
      "P.z[w] = Cx{cedm::cos((double)th), cedm::sin((double)th)};"),
-    ("sim reference sign", "const double u_raw = cedm::fma_(N, P.ref[t], -kx);", "const double u_raw = cedm::fma_(N, P.ref[t], kx);"),
+    ("sim reference sign", "      const double u_raw = cedm::fma_(N, P.ref[t], -kx);",
+     "      const double u_raw = cedm::fma_(N, P.ref[t], kx);"),
+    ("kalman gain denominator", "hphr = 1.0 / (hphr + R);", "hphr = 1.0 / hphr;"),
+    ("kalman innovation sign", "    dz = -dz;\n", "    dz = dz;\n"),
+    ("kalman process noise", "Pn[i * ns + i] = Pn[i * ns + i] + Q[i];", "Pn[i * ns + i] = Pn[i * ns + i];"),
+    ("kalman joseph term", "Pm[i * ns + j] = a + (kg[i] * R) * kg[j];", "Pm[i * ns + j] = a;"),
     ("sim observer innovation", "const double yerr = yv - yhat;", "const double yerr = yhat - yv;"),
 ]
 
