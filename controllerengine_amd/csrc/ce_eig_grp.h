@@ -37,6 +37,28 @@ namespace ce {
 
 __device__ __forceinline__ void grp_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// Maxima over the G lanes of a group (lanes gi G .. gi G + G - 1 of the wave, which run the same
+// control flow): exact and independent of the combination order.
+template <int G>
+__device__ __forceinline__ int grp_max_i(int v) {
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int o = __shfl_xor(v, d);
+    v = v > o ? v : o;
+  }
+  return v;
+}
+// v never NaN
+template <int G>
+__device__ __forceinline__ double grp_max_d(double v) {
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const double o = __shfl_xor(v, d);
+    v = (v < o) ? o : v;
+  }
+  return v;
+}
+
 struct GHh {
   double e0, e1, tau, beta;
 };
@@ -72,12 +94,15 @@ template <int G, int R = 1>
 __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l, int sign_block = -1,
                                        int* event = nullptr) {
   constexpr int S = G * R + 1;
+  // largest |a_ij| (NaN entries never win, as in the sequential fold): column maxima per lane,
+  // then the group maximum
   double scale = 0.0;
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) {
+  CE_LANES(j) if (j < n)
+    for (int i = 0; i < n; i++) {
       const double a = cedm::fabs_(T[i * S + j]);
       scale = (scale < a) ? a : scale;
     }
+  scale = grp_max_d<G>(scale);
   grp_sync();
   if (scale < DBL_MIN) {
     CE_LANES(j) if (j < n) { WR[j] = 0.0; WI[j] = 0.0; }
@@ -157,13 +182,19 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
   consider_zero = (consider_zero < DBL_MIN) ? DBL_MIN : consider_zero;
   if (norm != 0.0) {
     while (iu >= 0) {
-      int il = iu;
-      while (il > 0) {
-        double s = cedm::fabs_(T[(il - 1) * S + il - 1]) + cedm::fabs_(T[il * S + il]);
-        s = s * eps;
-        s = (s < consider_zero) ? consider_zero : s;
-        if (cedm::fabs_(T[il * S + il - 1]) <= s) break;
-        il--;
+      // findSmallSubdiagEntry: the largest il <= iu whose subdiagonal entry is negligible (0 when
+      // none is).  Every lane tests its own rows; the group maximum is the row the downward scan
+      // stops at.
+      int il;
+      {
+        int best = 0;
+        CE_LANES(r) if (r >= 1 && r <= iu) {
+          double s = cedm::fabs_(T[(r - 1) * S + r - 1]) + cedm::fabs_(T[r * S + r]);
+          s = s * eps;
+          s = (s < consider_zero) ? consider_zero : s;
+          if (cedm::fabs_(T[r * S + r - 1]) <= s && r > best) best = r;
+        }
+        il = grp_max_i<G>(best);
       }
       if (il == iu) {
         grp_sync();
@@ -257,22 +288,38 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
         iter++;
         total_iter++;
         if (total_iter > max_iters) return 1;
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        // initFrancisQRStep: the downward scan from iu - 2 stops at the first row whose start test
+        // passes, else at il.  Every lane tests its own rows in (il, iu - 2]; the group maximum of
+        // the passing rows is where the scan stops.  The scan tests the rows from iu - 2 down to that
+        // row, so a sign-flip event counts only there.
         int im;
-        for (im = iu - 2; im >= il; --im) {
+        {
+          int best = il, evm = -1;
+          CE_LANES(m) if (m >= il + 1 && m <= iu - 2) {
+            const double tmm = T[m * S + m];
+            const double r = sh0 - tmm;
+            const double s = sh1 - tmm;
+            const double a0 = (r * s - sh2) / T[(m + 1) * S + m] + T[m * S + m + 1];
+            const double a1 = T[(m + 1) * S + m + 1] - tmm - r - s;
+            const double a2 = T[(m + 2) * S + m + 1];
+            const double lhs = T[m * S + m - 1] * (cedm::fabs_(a1) + cedm::fabs_(a2));
+            const double rhs =
+                a0 * (cedm::fabs_(T[(m - 1) * S + m - 1]) + cedm::fabs_(tmm) + cedm::fabs_(T[(m + 1) * S + m + 1]));
+            const bool brk = cedm::fabs_(lhs) < eps * rhs;
+            if (brk && m > best) best = m;
+            if (event && m == sign_block - 1 && brk != (cedm::fabs_(lhs) < eps * -rhs)) evm = m;
+          }
+          im = grp_max_i<G>(best);
+          if (event && grp_max_i<G>(evm) >= im) *event = 1;
+        }
+        double v0, v1, v2;
+        {
           const double tmm = T[im * S + im];
           const double r = sh0 - tmm;
           const double s = sh1 - tmm;
           v0 = (r * s - sh2) / T[(im + 1) * S + im] + T[im * S + im + 1];
           v1 = T[(im + 1) * S + im + 1] - tmm - r - s;
           v2 = T[(im + 2) * S + im + 1];
-          if (im == il) break;
-          const double lhs = T[im * S + im - 1] * (cedm::fabs_(v1) + cedm::fabs_(v2));
-          const double rhs =
-              v0 * (cedm::fabs_(T[(im - 1) * S + im - 1]) + cedm::fabs_(tmm) + cedm::fabs_(T[(im + 1) * S + im + 1]));
-          if (event && im == sign_block - 1 && (cedm::fabs_(lhs) < eps * rhs) != (cedm::fabs_(lhs) < eps * -rhs))
-            *event = 1;
-          if (cedm::fabs_(lhs) < eps * rhs) break;
         }
         for (int k = im; k <= iu - 2; k++) {
           const bool first = (k == im);

@@ -2,7 +2,11 @@
 //
 // Stage map (reference file -> kernel):
 //   eigensetup + eigensolve + modal prelude   src/eigensetup.cu, src/eigensolve.cu, src/modalsetup.cu:8-50
-//        -> k_eig_fp64        one candidate per lane; LIN_A built in private memory, real-Schur QR.
+//        -> k_eig_lin<G,R>    G lanes per candidate (R rows each); LIN_A built in LDS, lane-group
+//                             real-Schur QR (ce_eig_grp.h), modes, dedupe, stability flags, and the
+//                             nominal check when the nominal loop is D LIN_A D
+//   allmargin nominal closed loop             src/allmargin.cu:61-79
+//        -> k_eig_nom<G,R>    the same solver on the candidates the LIN_A solve could not decide
 //   discrete_solve<Luenberger> + yss/overshoot   src/discrete_solves.cu:12-79, src/modalsetup.cu:52-62
 //        -> k_sim_fp64<NS,ND> one candidate per lane; state in VGPRs, plant/observer matrices read as
 //                             wave-uniform scalar loads; only the fit window of y is stored, SoA.
