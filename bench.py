@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3     # published (packed FP32); this build issues non-pa
 HBM_PEAK_GBS = 8000.0
 METRIC = "candidate evals/s, mixed vs FP64 plain at matched margins; p50/p99 job latency"
 
-F32_KERNELS = ("ce_sim_f32", "ce_sim_f32_g", "ce_sims_f32", "k_sim_f32", "k_sweep_mark32")
+F32_KERNELS = ("ce_sim_f32", "ce_sim_f32_g", "ce_sims_f32", "ce_sims_f32_g", "k_sim_f32", "k_sweep_mark32")
 WORKLOADS = {"c1": "mass-spring-damper 2-state", "c2": "two-mass drive 4-state, 2 NL",
              "c3": "random stable 8-state, 4 NL, Kalman observer"}
 
@@ -63,8 +63,8 @@ def kernel_flops(problem, info64, d, mixed):
     swept = int((info64[:, d.info_allmargin + 3] == 1).sum())
     return {
         "ce_sim_f32": sim, "ce_sim_f32_g": sim, "ce_sim_f64": sim, "ce_sim_f64_g": sim, "k_sim_f32": sim,
-        "ce_sims_f32": sim, "ce_sims_f64": sim,
-        "k_sim_fp64": sim,
+        "ce_sims_f32": sim, "ce_sims_f64": sim, "ce_sims_f32_g": sim, "ce_sims_f64_g": sim,
+        "k_sim_f64": sim,
         "k_eig_lin": float(10 * nt ** 3 + 4 * ns * ns + 2 * ns * nd + 2 * ns) * n,
         "k_eig_nom": float(10 * (2 * ns) ** 3) * n_sim,
         "k_atb": float(np.sum(8.0 * u * tfit)),
@@ -257,12 +257,17 @@ def bench_block(args):
     for name, prec in modes:
         for _ in range(args.warmup):
             eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
+        # the simulation kernel of every timed step, between HIP events on its launch stream
+        eng.set_sim_history(args.steps)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             eng.evaluate_device(cfg, plant, prec, g, i_in, i_out, i64)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        sim_hist = eng.sim_history()
+        sim_kernel = eng.last_sim_kernel()
+        eng.set_sim_history(0)
         info64 = i64.cpu().numpy()
         # per-kernel durations from HIP events recorded on the launch stream, separate passes
         eng.set_profiling(True)
@@ -273,7 +278,8 @@ def bench_block(args):
                 acc.setdefault(k, []).append(ms)
         eng.set_profiling(False)
         kern = {k: sum(v) / len(v) for k, v in acc.items() if k != "start"}
-        results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, kernels=kern, info64=info64)
+        results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, kernels=kern, info64=info64,
+                             sim_ms=sum(sim_hist) / len(sim_hist) if sim_hist else None, sim_kernel=sim_kernel)
         print("%s %s: %.4g candidate-evals/s, %.3f ms per step" % (args.config, name, results[name]["value"],
                                                                    results[name]["ms"]), file=sys.stderr, flush=True)
     head = modes[0][0]
@@ -282,6 +288,13 @@ def bench_block(args):
     kern = r["kernels"]
     dom = max(kern, key=lambda k: kern[k]) if kern else None
     dom_ms = kern.get(dom)
+    # the per-kernel passes run one stream; the timed steps overlap the simulation with the
+    # second stream's work, so the simulation kernel is timed over the timed steps themselves
+    ms_source = "per-kernel passes (one stream)"
+    if dom is not None and dom.startswith(("ce_sim", "k_sim")) and r["sim_ms"] is not None:
+        # the timed steps' form of the simulation (the gated form, where the chain starts it early)
+        dom, dom_ms = r["sim_kernel"], r["sim_ms"]
+        ms_source = "timed steps (HIP events on the launch stream)"
     peak = FP32_PEAK_TFLOPS if dom in F32_KERNELS else FP64_PEAK_TFLOPS
     achieved = fl[dom] / (dom_ms * 1e-3) / 1e12 if dom in fl else None
     traffic = None
@@ -299,7 +312,8 @@ def bench_block(args):
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         # vector FP32 / FP64 instructions only (no matrix op on this path): the compute roofline
         # is the VALU peak of the kernel's precision
-        "roofline": {"bound": "valu", "kernel": dom, "launch_ms": dom_ms, "achieved": achieved, "peak": peak,
+        "roofline": {"bound": "valu", "kernel": dom, "launch_ms": dom_ms, "launch_ms_source": ms_source,
+                     "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": (achieved / peak) if achieved is not None else None,
                      "traffic": traffic, "flops_per_launch": fl.get(dom)},
     }

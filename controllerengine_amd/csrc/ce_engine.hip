@@ -94,6 +94,7 @@ struct ce_engine {
   std::set<std::string> literal_structs;
   long long jit_evictions_seen = 0;
   int last_sim_jit = 0;
+  const char* last_sim_name = "";  // kernel name of the last simulation launch
   int last_mixed = 0;
   // mixed chain: the nominal-loop eigensolve and the sweep run on a second stream beside the
   // simulation and the fit (forked after the compaction, joined before the crossing scan)
@@ -103,6 +104,10 @@ struct ce_engine {
   Buf gate;
   int gate_gen = 0;
   int n_simd = 0;  // SIMDs of the device (4 per CU)
+  // simulation-stage history (ce_engine_set_sim_history): event pairs around the simulation kernel
+  // on the launch stream for the last hist_a.size() calls, a ring
+  std::vector<hipEvent_t> hist_a, hist_b;
+  int hist_pos = 0, hist_n = 0;
 };
 
 static void* ensure(ce_engine::Buf& b, size_t bytes) {
@@ -334,6 +339,8 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // stage 2: simulation (runtime-specialised unless disabled, else the interpreted kernel)
   void* ybuf = mixed ? (void*)B.ybuf32 : (void*)B.ybuf;
   E->last_sim_jit = 0;
+  const int hslot = E->hist_a.empty() ? -1 : E->hist_pos;
+  if (hslot >= 0) HIPCHK(hipEventRecord(E->hist_a[hslot], st));
   if (fn) {
     const bool st_form = E->sim_fn_struct[mixed ? 1 : 0][gated ? 1 : 0];
     bool ok;
@@ -346,11 +353,19 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
                                         (int*)E->gate.p, E->gate_gen)
                  : jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n);
     if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");
+    E->last_sim_name = st_form ? (mixed ? (gated ? "ce_sims_f32_g" : "ce_sims_f32") : (gated ? "ce_sims_f64_g" : "ce_sims_f64"))
+                               : (mixed ? (gated ? "ce_sim_f32_g" : "ce_sim_f32") : (gated ? "ce_sim_f64_g" : "ce_sim_f64"));
     kmark(st_form ? (mixed ? "ce_sims_f32" : "ce_sims_f64") : (mixed ? "ce_sim_f32" : "ce_sim_f64"), st);
     E->last_sim_jit = 1;
   } else {
     const bool ok = mixed ? launch_sim_mixed(ns, H.nd, P, B, st) : launch_sim_fp64(ns, H.nd, P, B, st);
+    E->last_sim_name = mixed ? "k_sim_f32" : "k_sim_f64";
     if (!ok) throw std::runtime_error("no simulation kernel for these dimensions");
+  }
+  if (hslot >= 0) {
+    HIPCHK(hipEventRecord(E->hist_b[hslot], st));
+    E->hist_pos = (hslot + 1) % (int)E->hist_a.size();
+    E->hist_n = std::min(E->hist_n + 1, (int)E->hist_a.size());
   }
   HIPCHK(hipEventRecord(E->ev[2], st));
   // stage 3: modal fit and behaviour score
@@ -405,6 +420,8 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (E->blob_pinned) hipHostFree(E->blob_pinned);
   jit_cache_destroy(E->jit);
   for (hipEvent_t ev : E->mark_ev) hipEventDestroy(ev);
+  for (hipEvent_t ev : E->hist_a) hipEventDestroy(ev);
+  for (hipEvent_t ev : E->hist_b) hipEventDestroy(ev);
   if (E->ev_fork) hipEventDestroy(E->ev_fork);
   if (E->ev_join) hipEventDestroy(E->ev_join);
   if (E->side) hipStreamDestroy(E->side);
@@ -478,6 +495,49 @@ extern "C" CE_EXPORT int ce_engine_stage_times(ce_engine* E, double* ms, int n_m
 
 // 1 when the last evaluation's simulation stage ran the runtime-specialised kernel, else 0.
 extern "C" CE_EXPORT int ce_engine_last_sim_jit(ce_engine* E) { return E ? E->last_sim_jit : 0; }
+
+extern "C" CE_EXPORT int ce_engine_last_sim_kernel(ce_engine* E, char* name, int len) {
+  if (!E || !name || len <= 0) return 1;
+  std::snprintf(name, (size_t)len, "%s", E->last_sim_name);
+  return 0;
+}
+
+// Simulation-stage history: `calls` event pairs (0 turns it off); durations of the recorded calls.
+extern "C" CE_EXPORT int ce_engine_set_sim_history(ce_engine* E, int calls) {
+  if (!E || calls < 0) return 1;
+  try {
+    HIPCHK(hipSetDevice(E->device));
+    HIPCHK(hipDeviceSynchronize());
+    for (hipEvent_t ev : E->hist_a) hipEventDestroy(ev);
+    for (hipEvent_t ev : E->hist_b) hipEventDestroy(ev);
+    E->hist_a.assign((size_t)calls, nullptr);
+    E->hist_b.assign((size_t)calls, nullptr);
+    for (int i = 0; i < calls; i++) {
+      HIPCHK(hipEventCreate(&E->hist_a[i]));
+      HIPCHK(hipEventCreate(&E->hist_b[i]));
+    }
+    E->hist_pos = 0;
+    E->hist_n = 0;
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
+extern "C" CE_EXPORT int ce_engine_sim_history(ce_engine* E, double* ms, int n_max) {
+  if (!E || E->hist_a.empty()) return 0;
+  const int cap = (int)E->hist_a.size();
+  const int first = (E->hist_pos - E->hist_n + cap) % cap;
+  int k = 0;
+  for (int i = 0; i < E->hist_n && k < n_max; i++, k++) {
+    const int s = (first + i) % cap;
+    float t = 0.0f;
+    if (hipEventSynchronize(E->hist_b[s]) != hipSuccess || hipEventElapsedTime(&t, E->hist_a[s], E->hist_b[s]) != hipSuccess)
+      return -1;
+    ms[k] = t;
+  }
+  return k;
+}
 
 // hipRTC compilations this engine has done.
 extern "C" CE_EXPORT long long ce_engine_jit_compiles(ce_engine* E) { return E ? jit_compiles(E->jit) : 0; }

@@ -108,6 +108,26 @@ class Engine:
         k = _abi.lib().ce_engine_stage_times(self.h, buf, 8)
         return [buf[i] for i in range(k)]
 
+    def last_sim_kernel(self):
+        """Kernel name of the last evaluation's simulation launch."""
+        buf = ctypes.create_string_buffer(64)
+        _abi.lib().ce_engine_last_sim_kernel(self.h, buf, 64)
+        return buf.value.decode()
+
+    def set_sim_history(self, calls):
+        """Record the simulation kernel's duration (HIP events on the launch stream) for the next
+        `calls` evaluations (a ring; 0 turns it off)."""
+        if _abi.lib().ce_engine_set_sim_history(self.h, int(calls)) != 0:
+            raise RuntimeError("ce_engine_set_sim_history failed")
+
+    def sim_history(self):
+        """Durations (ms) of the simulation kernels of the recorded evaluations, oldest first."""
+        buf = (ctypes.c_double * 4096)()
+        k = _abi.lib().ce_engine_sim_history(self.h, buf, 4096)
+        if k < 0:
+            raise RuntimeError("ce_engine_sim_history failed")
+        return [buf[i] for i in range(k)]
+
     def jit_compiles(self):
         """hipRTC compilations of this engine (simulation kernels)."""
         return int(_abi.lib().ce_engine_jit_compiles(self.h))

@@ -155,6 +155,8 @@ CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
 /* 1 when the last ce_evaluate ran the runtime-specialised (hipRTC) simulation kernel.  The engine
  * specialises the per-step code of a problem unless CE_JIT=0 is set in the environment. */
 CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
+/* Kernel name of the last call's simulation launch (as a kernel trace lists it). */
+CE_EXPORT int ce_engine_last_sim_kernel(ce_engine* eng, char* name, int len);
 /* hipRTC compilations this engine has done (no counterpart in the reference).  A problem is
  * compiled with its constants written into the source; once the engine sees a second problem of
  * the same dimensions and instruction structure (the rounds of a serial-mode run, a search over
@@ -162,6 +164,12 @@ CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
  * descriptor and uses that kernel for every later problem of the structure.  At most 32 kernels of
  * the first kind are kept (least recently used first out). */
 CE_EXPORT long long ce_engine_jit_compiles(ce_engine* eng);
+/* Simulation-stage history (no counterpart in the reference): with calls > 0, every ce_evaluate
+ * records HIP events on its stream around its simulation kernel, kept for the last `calls` calls
+ * (0 turns it off; the call waits for the device).  ce_engine_sim_history waits for the recorded
+ * events and returns up to n_max durations in ms, oldest first (the count, or -1). */
+CE_EXPORT int ce_engine_set_sim_history(ce_engine* eng, int calls);
+CE_EXPORT int ce_engine_sim_history(ce_engine* eng, double* ms, int n_max);
 /* Per-kernel timing (no counterpart in the reference).  With profiling on, ce_evaluate records an
  * event after every kernel it launches; ce_engine_kernel_times then returns, for the last call, the
  * number of kernels k (<= n_max), their durations in ms and their names, comma-separated, in

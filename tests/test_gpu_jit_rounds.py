@@ -43,3 +43,26 @@ def test_one_compile_per_structure_over_rounds(prec):
         assert counts[1] == counts[0] + 1 and counts[4] == counts[1], counts
     finally:
         eng.close()
+
+
+def test_sim_history_ring():
+    """The simulation-stage history keeps the last `calls` evaluations, oldest first, and recording
+    it changes no row."""
+    from controllerengine_amd.engine import Engine
+    p = plant()
+    eng = Engine(0)
+    try:
+        _, plain = eng.evaluate(p, precision=PRECISION_MIXED)
+        eng.set_sim_history(2)
+        assert eng.sim_history() == []
+        for r in range(3):
+            print("history call", r, flush=True)
+            _, got = eng.evaluate(p, precision=PRECISION_MIXED)
+            assert np.array_equal(got.view(np.uint64), plain.view(np.uint64))
+        h = eng.sim_history()
+        assert len(h) == 2 and all(ms > 0.0 for ms in h), h
+        eng.set_sim_history(0)
+        eng.evaluate(p, precision=PRECISION_MIXED)
+        assert eng.sim_history() == []
+    finally:
+        eng.close()
