@@ -91,21 +91,30 @@ def host_info():
     return model, avail
 
 
-def cpu_baseline(problem, target_s=12.0, one_core_s=4.0):
-    """The oracle (FP64 restatement of the reference chain, OpenMP over candidates) on a bounded
-    sample of the workload: on every host thread the process may use, then on one thread."""
+DRIFT_KEYS = ("max_dgm_db", "max_dpm_deg", "max_db", "max_dovershoot", "max_rel_dwn", "max_rel_dyss", "n_margin_rows",
+              "passfail_mismatch", "pass_rank_identical", "stable_mismatch", "behaviour_rel_err_median",
+              "behaviour_rel_err_p99", "score_rank_spearman")
+
+
+def cpu_baseline(problem, target_s=12.0, one_core_s=4.0, rows64=None, rows_mixed=None, drift_n=4096):
+    """The oracle's restatements of the reference chain (OpenMP over candidates) on a bounded sample
+    of the workload, on every host thread the process may use and on one thread: the plain FP64
+    chain and the ref-FP32 chain (the reference's own precision map).  With the GPU's FP64 and
+    mixed rows given, the ref-FP32 rows of the first drift_n candidates are compared with the FP64
+    rows beside the mixed rows of the same candidates: how far the reference's own arithmetic lies
+    from FP64, next to how far the mixed chain does."""
     import numpy as np
     from oracle import oracle
     model, avail = host_info()
     threads = int(os.environ.get("CE_CPU_THREADS", "0")) or min(avail, oracle.lib().ora_max_threads())
     n_total = problem.gains.shape[0]
 
-    def timed(nthreads, budget):
+    def timed(fn, nthreads, budget):
         n = min(256, n_total)
         while True:
             sub = problem.subset(np.arange(n))
             t0 = time.perf_counter()
-            oracle.evaluate(sub, threads=nthreads)
+            fn(sub, threads=nthreads)
             t = time.perf_counter() - t0
             if t >= budget / 3 or n >= n_total:
                 break
@@ -114,16 +123,31 @@ def cpu_baseline(problem, target_s=12.0, one_core_s=4.0):
             reps = max(1, int(budget / max(t, 1e-3)))
             t0 = time.perf_counter()
             for _ in range(reps):
-                oracle.evaluate(sub, threads=nthreads)
+                fn(sub, threads=nthreads)
             t = time.perf_counter() - t0
             return reps * n / t, "%d x %d candidates, %.1f s" % (reps, n, t)
         return n / t, "first %d of %d candidates, %.1f s" % (n, n_total, t)
 
-    v_all, s_all = timed(threads, target_s)
-    v_one, s_one = timed(1, one_core_s)
-    return dict(value=v_all, unit="candidate-evals/s", cores=threads, kind="port",
-                sample="FP64 oracle, all six stages: " + s_all, one_core_value=v_one, one_core_sample=s_one,
-                host_cpu=model, host_threads_available=avail)
+    v_all, s_all = timed(oracle.evaluate, threads, target_s)
+    v_one, s_one = timed(oracle.evaluate, 1, one_core_s)
+    r_all, rs_all = timed(oracle.evaluate_ref32, threads, target_s / 2)
+    r_one, rs_one = timed(oracle.evaluate_ref32, 1, one_core_s / 2)
+    out = dict(value=v_all, unit="candidate-evals/s", cores=threads, kind="port",
+               sample="FP64 oracle, all six stages: " + s_all, one_core_value=v_one, one_core_sample=s_one,
+               ref32_value=r_all, ref32_sample="ref-FP32 oracle (the reference's precision map): " + rs_all,
+               ref32_one_core_value=r_one, ref32_one_core_sample=rs_one,
+               host_cpu=model, host_threads_available=avail)
+    if rows64 is not None:
+        from controllerengine_amd import metrics
+        k = min(drift_n, n_total)
+        sub = problem.subset(np.arange(k))
+        r32, _ = oracle.evaluate_ref32(sub, threads=threads)
+        drift = {"n": k, "ref32_vs_fp64": metrics.compare(sub, rows64[:k], r32.astype(np.float64))}
+        if rows_mixed is not None:
+            drift["mixed_vs_fp64"] = metrics.compare(sub, rows64[:k], rows_mixed[:k])
+        out["precision_drift_sample"] = {
+            key: ({kk: v[kk] for kk in DRIFT_KEYS if kk in v} if isinstance(v, dict) else v) for key, v in drift.items()}
+    return out
 
 
 def bench_c4(args, steps=None, fp64_steps=None):
@@ -335,7 +359,9 @@ def bench_block(args):
     line["host_buffers_value"] = n * hs / (time.perf_counter() - t0)
     eng.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(problem, target_s=args.cpu_seconds)
+        rows64 = results["fp64"]["info64"] if "fp64" in results else None
+        rows_mixed = results["mixed"]["info64"] if "mixed" in results else None
+        line["cpu_baseline"] = cpu_baseline(problem, target_s=args.cpu_seconds, rows64=rows64, rows_mixed=rows_mixed)
     return line
 
 

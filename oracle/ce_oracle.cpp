@@ -13,6 +13,9 @@
 //   modalsolve              src/modalsolve.cu:1-108        (Eigen LDLT, restated, in double)
 //   allmargin               src/allmargin.cu:1-200         (Eigen PartialPivLU<complex>, restated, in double)
 //   orchestrate_sims        src/main.cu:402-582 (one round)
+// and, apart (namespace r32, ora_evaluate_ref32), the same chain in the reference's own precision
+// map: float simulation with hi/lo state, float eigensetup, double eigensolve stored as float,
+// float modal fit and LDLT, complex<float> margin sweep.
 //
 // Third-party algorithms restated (Eigen, version unpinned: lib/eigen is an empty submodule):
 //   EigenSolver/RealSchur: Householder Hessenberg reduction, Francis double-shift QR with the
@@ -477,10 +480,21 @@ int eigvals(double* A, int n, double* wr, double* wi, int sign_block = -1, int* 
 
 // ---------------------------------------------------------------------------------------------
 // LDLT<Lower> with symmetric pivoting, restated; solves in place.  M is n x n row-major, full
-// symmetric (only the lower triangle is read).
-void ldlt_solve(double* M, int n, double* x) {
+// symmetric (only the lower triangle is read).  R = double (the plain chain) or float (the
+// reference's own modalsolve.cu:73-74, MatF).
+inline double fmaR(double a, double b, double c) { return cedm::fma_(a, b, c); }
+inline float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <typename R>
+R min_normal();
+template <>
+double min_normal<double>() { return DBL_MIN; }
+template <>
+float min_normal<float>() { return FLT_MIN; }
+
+template <typename R>
+void ldlt_solve_t(R* M, int n, R* x) {
   int trans[2 * CE_MAX_TOTAL_DIM];
-  double temp[2 * CE_MAX_TOTAL_DIM];
+  R temp[2 * CE_MAX_TOTAL_DIM];
   bool all_zero = false;
   for (int k = 0; k < n; k++) {
     int idx = k;
@@ -503,16 +517,16 @@ void ldlt_solve(double* M, int n, double* x) {
     const int rs = n - k - 1;
     if (k > 0) {
       for (int j = 0; j < k; j++) temp[j] = M[j * n + j] * M[k * n + j];
-      double acc = 0.0;
-      for (int j = 0; j < k; j++) acc = cedm::fma_(M[k * n + j], temp[j], acc);
+      R acc = R(0);
+      for (int j = 0; j < k; j++) acc = fmaR(M[k * n + j], temp[j], acc);
       M[k * n + k] = M[k * n + k] - acc;
       for (int i = k + 1; i < n; i++) {
-        double a = 0.0;
-        for (int j = 0; j < k; j++) a = cedm::fma_(M[i * n + j], temp[j], a);
+        R a = R(0);
+        for (int j = 0; j < k; j++) a = fmaR(M[i * n + j], temp[j], a);
         M[i * n + k] = M[i * n + k] - a;
       }
     }
-    const double akk = M[k * n + k];
+    const R akk = M[k * n + k];
     const bool valid = cedm::fabs_(akk) > 0.0;
     if (k == 0 && !valid) {
       all_zero = true;
@@ -522,59 +536,66 @@ void ldlt_solve(double* M, int n, double* x) {
       for (int i = k + 1; i < n; i++) M[i * n + k] = M[i * n + k] / akk;
   }
   if (all_zero) {
-    for (int i = 0; i < n; i++) x[i] = 0.0;
+    for (int i = 0; i < n; i++) x[i] = R(0);
     return;
   }
   for (int k = 0; k < n; k++) std::swap(x[k], x[trans[k]]);
   for (int i = 0; i < n; i++) {
-    double acc = 0.0;
-    for (int j = 0; j < i; j++) acc = cedm::fma_(M[i * n + j], x[j], acc);
+    R acc = R(0);
+    for (int j = 0; j < i; j++) acc = fmaR(M[i * n + j], x[j], acc);
     x[i] = x[i] - acc;
   }
   for (int i = 0; i < n; i++) {
-    const double d = M[i * n + i];
-    if (cedm::fabs_(d) > DBL_MIN) x[i] = x[i] / d;
-    else x[i] = 0.0;
+    const R d = M[i * n + i];
+    if (cedm::fabs_(d) > min_normal<R>()) x[i] = x[i] / d;
+    else x[i] = R(0);
   }
   for (int i = n - 1; i >= 0; i--) {
-    double acc = 0.0;
-    for (int j = i + 1; j < n; j++) acc = cedm::fma_(M[j * n + i], x[j], acc);
+    R acc = R(0);
+    for (int j = i + 1; j < n; j++) acc = fmaR(M[j * n + i], x[j], acc);
     x[i] = x[i] - acc;
   }
   for (int k = n - 1; k >= 0; k--) std::swap(x[k], x[trans[k]]);
 }
+void ldlt_solve(double* M, int n, double* x) { ldlt_solve_t<double>(M, n, x); }
 
 // ---------------------------------------------------------------------------------------------
 // Complex partial-pivot LU solve of (z I - A) x = b with A real n x n (row-major) and b real.
 // Returns false when x is not finite (Eigen's allFinite check in allmargin.cu:27,105).
-struct Cx {
-  double r, i;
+template <typename R>
+struct CxT {
+  R r, i;
 };
-inline Cx cmul(Cx a, Cx b) { return Cx{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-inline Cx cdiv(Cx a, Cx b) {
-  const double d = b.r * b.r + b.i * b.i;
-  return Cx{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+using Cx = CxT<double>;
+template <typename R>
+inline CxT<R> cmul(CxT<R> a, CxT<R> b) { return CxT<R>{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+template <typename R>
+inline CxT<R> cdiv(CxT<R> a, CxT<R> b) {
+  const R d = b.r * b.r + b.i * b.i;
+  return CxT<R>{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
 
-bool shifted_solve(const double* A, int n, Cx z, const double* b, Cx* x) {
+template <typename R>
+bool shifted_solve_t(const R* A, int n, CxT<R> z, const R* b, CxT<R>* x) {
   // Gaussian elimination with partial pivoting on [zI - A | b]: row k of the pivot search is the
   // first row of largest squared modulus; the right-hand side is eliminated with the same
   // multipliers in the same order as a column-oriented forward substitution, and the back
   // substitution is column-oriented (x_i = v_i / U_ii, then v_j -= U_ji x_i for j < i).
-  Cx M[CE_MAX_STATESPACE_DIM][CE_MAX_STATESPACE_DIM];
-  Cx v[CE_MAX_STATESPACE_DIM];
+  using C = CxT<R>;
+  C M[CE_MAX_STATESPACE_DIM][CE_MAX_STATESPACE_DIM];
+  C v[CE_MAX_STATESPACE_DIM];
   for (int i = 0; i < n; i++) {
-    for (int j = 0; j < n; j++) M[i][j] = Cx{(i == j ? z.r : 0.0) - A[i * n + j], (i == j ? z.i : 0.0)};
-    v[i] = Cx{b[i], 0.0};
+    for (int j = 0; j < n; j++) M[i][j] = C{(i == j ? z.r : R(0)) - A[i * n + j], (i == j ? z.i : R(0))};
+    v[i] = C{b[i], R(0)};
   }
   for (int k = 0; k < n; k++) {
     int p = k;
-    double best = M[k][k].r * M[k][k].r + M[k][k].i * M[k][k].i;
+    R best = M[k][k].r * M[k][k].r + M[k][k].i * M[k][k].i;
     for (int i = k + 1; i < n; i++) {
-      const double s = M[i][k].r * M[i][k].r + M[i][k].i * M[i][k].i;
+      const R s = M[i][k].r * M[i][k].r + M[i][k].i * M[i][k].i;
       if (s > best) { best = s; p = i; }
     }
-    if (best != 0.0) {
+    if (best != R(0)) {
       if (p != k) {
         for (int j = k; j < n; j++) std::swap(M[k][j], M[p][j]);
         std::swap(v[k], v[p]);
@@ -583,24 +604,25 @@ bool shifted_solve(const double* A, int n, Cx z, const double* b, Cx* x) {
     }
     for (int i = k + 1; i < n; i++) {
       for (int j = k + 1; j < n; j++) {
-        const Cx pr = cmul(M[i][k], M[k][j]);
-        M[i][j] = Cx{M[i][j].r - pr.r, M[i][j].i - pr.i};
+        const C pr = cmul(M[i][k], M[k][j]);
+        M[i][j] = C{M[i][j].r - pr.r, M[i][j].i - pr.i};
       }
-      const Cx pv = cmul(M[i][k], v[k]);
-      v[i] = Cx{v[i].r - pv.r, v[i].i - pv.i};
+      const C pv = cmul(M[i][k], v[k]);
+      v[i] = C{v[i].r - pv.r, v[i].i - pv.i};
     }
   }
   for (int i = n - 1; i >= 0; i--) {
     x[i] = cdiv(v[i], M[i][i]);
     for (int j = 0; j < i; j++) {
-      const Cx pr = cmul(M[j][i], x[i]);
-      v[j] = Cx{v[j].r - pr.r, v[j].i - pr.i};
+      const C pr = cmul(M[j][i], x[i]);
+      v[j] = C{v[j].r - pr.r, v[j].i - pr.i};
     }
   }
   bool fin = true;
-  for (int i = 0; i < n; i++) fin = fin && cedm::isfinite_(x[i].r) && cedm::isfinite_(x[i].i);
+  for (int i = 0; i < n; i++) fin = fin && cedm::isfinite_((double)x[i].r) && cedm::isfinite_((double)x[i].i);
   return fin;
 }
+bool shifted_solve(const double* A, int n, Cx z, const double* b, Cx* x) { return shifted_solve_t<double>(A, n, z, b, x); }
 
 // real-weighted dot sum_i w_i x_i of a complex vector
 inline Cx rdot(const double* w, const Cx* x, int n) {
@@ -628,6 +650,12 @@ struct Problem {
   std::vector<Cx> presp;
   std::vector<char> pok;
   double alpha;
+  // the same plant response and scaling in complex<float> (the reference's own arithmetic,
+  // allmargin.cu:21-41), for the ref-FP32 chain
+  std::vector<CxT<float>> presp32;
+  std::vector<char> pok32;
+  float alpha32 = 0.0f;
+  bool alpha32_ok = false;
 };
 
 bool setup(const ce_config& c, const ce_plant& pl, Problem& P, std::string& err) {
@@ -705,6 +733,34 @@ bool setup(const ce_config& c, const ce_plant& pl, Problem& P, std::string& err)
       return false;
     }
     P.alpha = 1.0 / pmag;
+  }
+  {  // float forms (allmargin.cu:21-41 as written: complex<float> LU solves, std::abs in float)
+    float pa[64], pb[8], pc[8];
+    for (int i = 0; i < ns * ns; i++) pa[i] = (float)P.pa[i];
+    for (int i = 0; i < ns; i++) { pb[i] = (float)P.pb[i]; pc[i] = (float)P.pc[i]; }
+    const float pd = (float)P.pd;
+    auto resp32 = [&](CxT<float> z, CxT<float>& g) {
+      CxT<float> x[CE_MAX_STATESPACE_DIM];
+      const bool ok = shifted_solve_t<float>(pa, ns, z, pb, x);
+      CxT<float> acc{0.0f, 0.0f};
+      for (int i = 0; i < ns; i++) acc = CxT<float>{acc.r + pc[i] * x[i].r, acc.i + pc[i] * x[i].i};
+      g = CxT<float>{acc.r + pd, acc.i};
+      return ok;
+    };
+    P.presp32.resize(F);
+    P.pok32.resize(F);
+    for (int w = 0; w < F; w++) {
+      CxT<float> g;
+      P.pok32[w] = resp32(CxT<float>{(float)P.z[w].r, (float)P.z[w].i}, g);
+      P.presp32[w] = P.pok32[w] ? g : CxT<float>{0.0f, 0.0f};
+    }
+    float s, co;
+    cedm::sincosf_(P.d.crossover * P.d.ts, s, co);
+    CxT<float> g;
+    const bool ok = resp32(CxT<float>{co, s}, g);
+    const float pmag = (float)cedm::hypot((double)g.r, (double)g.i);
+    P.alpha32_ok = ok && pmag > 0.0f && cedm::isfinite_((double)pmag);
+    P.alpha32 = P.alpha32_ok ? 1.0f / pmag : 0.0f;
   }
   return true;
 }
@@ -1030,6 +1086,391 @@ void run_candidate(const Problem& P, const float* gains, double* info, const Dia
   sc[0] = cedm::isfinite_(total) ? total : EF;
 }
 
+// =============================================================================================
+// ref-FP32 chain: the reference's own precision map, restated for the drift statement (how far
+// the reference's arithmetic lies from the plain FP64 chain) and as the bit-exact check of the
+// product's float simulation.
+//   discrete_solve<Luen.>  src/discrete_solves.cu:12-79   float, hi/lo state (hladdf), fmaf chains
+//   eigensetup             src/eigensetup.cu:1-57         float products and differences
+//   eigensolve             src/eigensolve.cu:1-19         double solve of the float matrix, stored float
+//   modalsetup             src/modalsetup.cu:1-97         float modes, statistics and Gram sums
+//   modalsolve             src/modalsolve.cu:1-108        float scaling and LDLT (MatF)
+//   allmargin              src/allmargin.cu:1-200         complex<float> LU sweep; double nominal solve
+// The device stages (simulation, eigensetup, modalsetup) use fma where the reference's source
+// calls __fmaf_rn or a product feeds a sum of the same expression; the host stages (modalsolve,
+// allmargin) evaluate as written.  Float elementary functions are the double ones of ce_detmath.h
+// rounded to float, and Eigen's complex reductions are taken in index order, so this is the
+// reference's precision map rather than its bits (nvcc's logf / Eigen's packet order differ).
+namespace r32 {
+constexpr float PIF = 0x1.921fb6p+1f;          // src/main.cu:41
+constexpr float EPS30F = 0x1.a484cp-100f;
+constexpr float EPS6F = 0x1.0c6f7ap-20f;
+constexpr float BIG10F = 0x1.2a05f2p+33f;
+constexpr float FLTMAXF = 0x1.fffffep127f;
+inline float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+inline float fminf_(float a, float b) { return __builtin_fminf(a, b); }
+inline float fmaxf_(float a, float b) { return __builtin_fmaxf(a, b); }
+inline float fabsf_(float a) { return __builtin_fabsf(a); }
+inline float sqrtf_(float a) { return __builtin_sqrtf(a); }
+inline float floorf_(float a) { return __builtin_floorf(a); }
+inline float hypotf_(float a, float b) { return (float)cedm::hypot((double)a, (double)b); }
+inline float log10f_(float a) { return (float)cedm::log10((double)a); }
+inline bool finite(float a) { return cedm::isfinite_((double)a); }
+
+// the interpreter in float (src/main.cu:260-291)
+float interp(const Ins* ins, int count, const float* src, bool has_seed, float seed) {
+  float result = 0.0f;
+  bool started = false;
+  for (int r = 0; r < count; r++) {
+    const Ins& in = ins[r];
+    if (in.op == 0) continue;
+    const float pos = src[in.col];
+    const float prev = started ? result : ((has_seed && seed == seed) ? seed : pos);
+    const float a = in.akind == K_CONST ? (float)in.aval : (in.akind == K_STATE ? pos : prev);
+    const float b = in.bkind == K_CONST ? (float)in.bval : (in.bkind == K_STATE ? pos : prev);
+    switch (in.op) {
+      case 1: result = a + b; break;
+      case 2: result = a - b; break;
+      case 3: result = a * b; break;
+      case 4: result = a / b; break;
+      case 5: result = fminf_(fmaxf_(prev, a), b); break;
+      case 6: result = (float)((0.0f < a) - (a < 0.0f)); break;
+      default: break;
+    }
+    started = true;
+  }
+  return result;
+}
+
+// hladdf (src/main.cu:129-155): TwoSum of the hi parts, lo parts plus the error, Fast2Sum
+inline void hladd(float& HI, float& LO, float h, float l) {
+  const float sh = HI + h;
+  const float b0 = sh - HI;
+  const float sl = (HI - (sh - b0)) + (h - b0);
+  const float v = (LO + l) + sl;
+  HI = sh + v;
+  LO = v - (HI - sh);
+}
+
+// discrete_solve<Luenberger> in float (src/discrete_solves.cu:12-79)
+void simulate(const Problem& P, const float* gains, float* y) {
+  const ce_config& c = P.cfg;
+  const int ns = P.ns, nd = P.nd, nw = P.nw, T = c.time;
+  float K[8], L[8], sai[64], sb[8], scv[8], dai[64], dbu[8], dbw[32];
+  for (int i = 0; i < ns; i++) { K[i] = gains[i]; L[i] = gains[ns + i]; }
+  const float N = gains[2 * ns];
+  for (int i = 0; i < ns * ns; i++) sai[i] = (float)P.sai[i];
+  for (int i = 0; i < ns; i++) { sb[i] = (float)P.sb[i]; scv[i] = (float)P.sc[i]; }
+  const float sd = (float)P.sd;
+  for (int i = 0; i < nd * nd; i++) dai[i] = (float)P.dai[i];
+  for (int i = 0; i < nd; i++) dbu[i] = (float)P.dbu[i];
+  for (int i = 0; i < nd * nw; i++) dbw[i] = (float)P.dbw[i];
+  float xh[8] = {0}, xhl[8] = {0}, xp[8] = {0}, xpl[8] = {0};
+  for (int t = 0; t < T; t++) {
+    const float yv = interp(P.dc_ops, c.dc_instructions, xp, false, 0.0f);
+    y[t] = yv;
+    float kx = 0.0f;
+    for (int k = 0; k < ns; k++) kx = fmaf_(K[k], xh[k], kx);
+    for (int k = 0; k < ns; k++) kx = fmaf_(K[k], xhl[k], kx);
+    const float u_raw = fmaf_(N, (float)P.ref[t], -kx);
+    const float u = interp(P.u_ops, c.u_instructions, xh, true, u_raw);
+    float cx = 0.0f;
+    for (int k = 0; k < ns; k++) cx = fmaf_(scv[k], xh[k], cx);
+    for (int k = 0; k < ns; k++) cx = fmaf_(scv[k], xhl[k], cx);
+    const float yerr = yv - fmaf_(sd, u, cx);
+    float dh[8], dl[8];
+    for (int i = 0; i < ns; i++) {
+      float acc = fmaf_(L[i], yerr, sb[i] * u);
+      for (int k = 0; k < ns; k++) acc = fmaf_(sai[i * ns + k], xh[k], acc);
+      dh[i] = acc;
+      float lo = 0.0f;
+      for (int k = 0; k < ns; k++) lo = fmaf_(sai[i * ns + k], xhl[k], lo);
+      dl[i] = lo;
+    }
+    for (int i = 0; i < ns; i++) hladd(xh[i], xhl[i], dh[i], dl[i]);
+    float nls[CE_MAX_NL_DIM];
+    for (int i = 0; i < nd; i++) nls[i] = xp[i];
+    nls[nd] = yv;
+    float W[4];
+    for (int q = 0; q < nw; q++) W[q] = interp(P.nl_ops[q], P.nl_count[q], nls, false, 0.0f);
+    float ph[8], pl[8];
+    for (int i = 0; i < nd; i++) {
+      float base = dbu[i] * u;
+      for (int q = 0; q < nw; q++) base = fmaf_(dbw[i * nw + q], W[q], base);
+      float acc = base;
+      for (int k = 0; k < nd; k++) acc = fmaf_(dai[i * nd + k], xp[k], acc);
+      ph[i] = acc;
+      float lo = 0.0f;
+      for (int k = 0; k < nd; k++) lo = fmaf_(dai[i * nd + k], xpl[k], lo);
+      pl[i] = lo;
+    }
+    for (int i = 0; i < nd; i++) hladd(xp[i], xpl[i], ph[i], pl[i]);
+  }
+}
+
+// continuous-time modes of the float eigenvalues (modalsetup.cu:8-22, modalsolve.cu:21-33)
+float modes(const float* wr, const float* wi, int nt, float ts, float* Re, float* Im) {
+  float max_mag = 0.0f;
+  for (int i = 0; i < nt; i++) {
+    const float mag = hypotf_(wr[i], wi[i]);
+    max_mag = fmaxf_(max_mag, mag);
+    Re[i] = cedm::logf_(fmaxf_(mag, EPS30F)) / ts;
+    Im[i] = cedm::atan2f_(wi[i], wr[i]) / ts;
+  }
+  return max_mag;
+}
+
+// One candidate; info: info_w floats in place; y: time floats (the trajectory).
+void candidate(const Problem& P, const float* gains, float* info, float* y) {
+  const ce_config& c = P.cfg;
+  const Derived& D = P.d;
+  const int ns = P.ns, nd = P.nd, nt = P.nt, T = c.time;
+  float K[8], L[8];
+  for (int i = 0; i < ns; i++) { K[i] = gains[i]; L[i] = gains[ns + i]; }
+  simulate(P, gains, y);
+
+  // eigensetup (float): A - B K - L (C - D K), L C_d, -B_u K, A_d
+  float lina[CE_MAX_TOTAL_DIM * CE_MAX_TOTAL_DIM];
+  for (int i = 0; i < ns; i++) {
+    for (int j = 0; j < ns; j++) {
+      const float bk = (float)P.sb[i] * K[j];
+      const float cdk = (float)P.sc[j] - (float)P.sd * K[j];
+      const float lcd = L[i] * cdk;
+      lina[i * nt + j] = ((float)P.sa[i * ns + j] - bk) - lcd;
+    }
+    for (int j = 0; j < nd; j++) lina[i * nt + ns + j] = L[i] * (float)P.dc[j];
+  }
+  for (int i = 0; i < nd; i++) {
+    for (int j = 0; j < ns; j++) lina[(ns + i) * nt + j] = -((float)P.dbu[i] * K[j]);
+    for (int j = 0; j < nd; j++) lina[(ns + i) * nt + ns + j] = (float)P.da[i * nd + j];
+  }
+  // eigensolve: double solve of the float matrix, eigenvalues stored as float
+  float wr[CE_MAX_TOTAL_DIM], wi[CE_MAX_TOTAL_DIM];
+  {
+    double A[CE_MAX_TOTAL_DIM * CE_MAX_TOTAL_DIM], er[CE_MAX_TOTAL_DIM], ei[CE_MAX_TOTAL_DIM];
+    for (int i = 0; i < nt * nt; i++) A[i] = lina[i];
+    const bool ok = eigvals(A, nt, er, ei) == 0;
+    for (int i = 0; i < nt; i++) {
+      wr[i] = ok ? (float)er[i] : (float)cedm::qnan();
+      wi[i] = ok ? (float)ei[i] : (float)cedm::qnan();
+    }
+  }
+  float* sc = info + D.info_scoring;
+  float* fr = info + D.info_freqresp;
+  float* am = info + D.info_allmargin;
+  float* md = info + D.info_modal;
+  const float ts = D.ts;
+  float Re[CE_MAX_TOTAL_DIM], Im[CE_MAX_TOTAL_DIM];
+  const float max_mag = modes(wr, wi, nt, ts, Re, Im);
+  float mr[2 * CE_MAX_TOTAL_DIM];
+  int u = 0;
+  {
+    float pre = BIG10F, pim = BIG10F;
+    for (int i = 0; i < nt; i++) {
+      const bool dup = fabsf_(Re[i] - pre) < EPS6F && fabsf_(fabsf_(Im[i]) - fabsf_(pim)) < EPS6F;
+      pre = Re[i];
+      pim = Im[i];
+      if (!dup) { mr[2 * u] = Re[i]; mr[2 * u + 1] = fabsf_(Im[i]); u++; }
+    }
+  }
+  const int nc2 = 2 * nt;
+  float AtA[4 * CE_MAX_TOTAL_DIM * CE_MAX_TOTAL_DIM] = {0};
+  float Atb[2 * CE_MAX_TOTAL_DIM] = {0};
+
+  // modalsetup (float)
+  if (max_mag > 1.0f) {
+    sc[0] = c.unstable_err + cedm::logf_(max_mag) / ts;
+    sc[1] = sc[0];
+  } else {
+    float shift[CE_MAX_TOTAL_DIM];
+    for (int e = 0; e < u; e++) shift[e] = fmaxf_(mr[2 * e] * D.fit_t0, mr[2 * e] * D.fit_t1);
+    float yss = 0.0f;
+    for (int t = D.steady_start_id; t < T; t++) yss = yss + y[t];
+    yss = yss / (float)(T - D.steady_start_id);
+    float peak = 0.0f;
+    for (int t = 0; t < T; t++) peak = fmaxf_(peak, y[t] * yss);
+    fr[2] = peak / fmaxf_(yss * yss, EPS6F) - 1.0f;
+    fr[3] = yss;
+    const int nc = 2 * u;
+    float cf[2 * CE_MAX_TOTAL_DIM];
+    for (int t = 0; t < D.fit_count; t++) {
+      const float tval = (float)(D.fit_start_id + t) * ts;
+      for (int e = 0; e < u; e++) {
+        const float decay = cedm::expf_(fmaf_(mr[2 * e], tval, -shift[e]));
+        float s, co;
+        cedm::sincosf_(mr[2 * e + 1] * tval, s, co);
+        cf[2 * e] = decay * co;
+        cf[2 * e + 1] = decay * s;
+      }
+      const float yv = y[D.fit_start_id + t] - yss;
+      for (int i = 0; i < nc; i++) {
+        for (int j = 0; j <= i; j++) AtA[i * nc2 + j] = fmaf_(cf[i], cf[j], AtA[i * nc2 + j]);
+        Atb[i] = fmaf_(cf[i], yv, Atb[i]);
+      }
+    }
+  }
+
+  // modalsolve (float, host: as written)
+  if (sc[0] == 0.0f) {
+    float cs[2 * CE_MAX_TOTAL_DIM], S[4 * CE_MAX_TOTAL_DIM * CE_MAX_TOTAL_DIM], xs[2 * CE_MAX_TOTAL_DIM];
+    for (int i = 0; i < nc2; i++) {
+      const float s = sqrtf_(AtA[i * nc2 + i]);
+      cs[i] = (s > EPS6F) ? s : 1.0f;
+    }
+    for (int i = 0; i < nc2; i++) {
+      const float inv_i = 1.0f / cs[i];
+      for (int j = 0; j < nc2; j++) {
+        const float inv_j = 1.0f / cs[j];
+        const float m = (i >= j) ? AtA[i * nc2 + j] : AtA[j * nc2 + i];
+        S[i * nc2 + j] = (inv_i * m) * inv_j;
+      }
+      xs[i] = Atb[i] / cs[i];
+    }
+    ldlt_solve_t<float>(S, nc2, xs);
+    float coef[2 * CE_MAX_TOTAL_DIM];
+    for (int i = 0; i < nc2; i++) coef[i] = xs[i] / cs[i];
+    float esum = 0.0f, lw = 0.0f, zw = 0.0f;
+    for (int e = 0; e < u; e++) {
+      const int id = 2 * e;
+      const float c0 = coef[id], c1 = coef[id + 1];
+      const float g00 = AtA[id * nc2 + id], g01 = AtA[(id + 1) * nc2 + id], g11 = AtA[(id + 1) * nc2 + id + 1];
+      const float energy = ((c0 * c0) * g00 + ((2.0f * c0) * c1) * g01) + (c1 * c1) * g11;
+      const float re_e = mr[id], im_e = mr[id + 1];
+      const float wn_e = fmaxf_(sqrtf_(re_e * re_e + im_e * im_e), EPS6F);
+      md[id] = wn_e;
+      md[id + 1] = -re_e / wn_e;
+      md[2 * nt + e] = energy;
+      if (im_e < EPS6F) continue;
+      esum = esum + energy;
+      lw = lw + energy * cedm::logf_(wn_e);
+      zw = zw + energy * (-re_e / wn_e);
+    }
+    md[3 * nt] = (float)u;
+    float wn, zeta;
+    if (esum <= EPS6F) { wn = EPS6F; zeta = 0.0f; }
+    else { wn = cedm::expf_(lw / esum); zeta = zw / esum; }
+    const float dmp = c.damping, d2 = dmp * dmp;
+    const float twn = ((2.0f * PIF) * c.bandwidth) / sqrtf_((1.0f - 2.0f * d2) + sqrtf_(((4.0f * d2) * d2 - 4.0f * d2) + 2.0f));
+    const float lr = cedm::logf_(wn / twn);
+    float beh = lr * lr + ((zeta - dmp) * (zeta - dmp) / dmp) / dmp;
+    const float err_os = fmaxf_(0.0f, fr[2] - c.overshoot) / c.overshoot;
+    beh = beh + err_os * err_os;
+    const float err_ss = fmaxf_(0.0f, fabsf_(fr[3] / c.ref_amplitude - 1.0f) - c.steady_state_error) / c.steady_state_error;
+    beh = beh + err_ss * err_ss;
+    sc[0] = finite(beh) ? beh : c.eig_fail_err;
+    sc[1] = sc[0];
+    fr[0] = wn;
+    fr[1] = zeta;
+  }
+
+  // allmargin (complex<float> sweep; double nominal eigensolve of the float closed loop)
+  const float UE = c.unstable_err, EF = c.eig_fail_err;
+  if (sc[0] >= UE) {
+    sc[1] = sc[0]; sc[2] = sc[0];
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0f;
+    return;
+  }
+  float oa[64], pa[64], pb[8], pc[8];
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < ns; j++) oa[i * ns + j] = lina[i * nt + j];
+  for (int i = 0; i < ns * ns; i++) pa[i] = (float)P.pa[i];
+  for (int i = 0; i < ns; i++) { pb[i] = (float)P.pb[i]; pc[i] = (float)P.pc[i]; }
+  const float pd = (float)P.pd;
+  bool nominal_stable;
+  {
+    const int m = 2 * ns;
+    double Acl[4 * 64], er[16], ei[16];
+    for (int i = 0; i < ns; i++)
+      for (int j = 0; j < ns; j++) {
+        Acl[i * m + j] = oa[i * ns + j] - pd * (L[i] * K[j]);
+        Acl[i * m + ns + j] = -(L[i] * pc[j]);
+        Acl[(ns + i) * m + j] = pb[i] * K[j];
+        Acl[(ns + i) * m + ns + j] = pa[i * ns + j];
+      }
+    nominal_stable = eigvals(Acl, m, er, ei) == 0;
+    if (nominal_stable)
+      for (int i = 0; i < m; i++)
+        if (cedm::hypot(er[i], ei[i]) >= 1.0) { nominal_stable = false; break; }
+  }
+  if (!nominal_stable) {
+    sc[1] = sc[0];
+    sc[2] = UE;
+    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0f;
+    const float total = cedm::logf_((sc[0] + UE) + EPS30F);
+    sc[0] = finite(total) ? total : EF;
+    return;
+  }
+  using C = CxT<float>;
+  const float alpha = P.alpha32;
+  float PM = FLTMAXF, GM = FLTMAXF, DM = FLTMAXF, B = 1.0f;
+  bool crossed = false, ok = true;
+  float prevMag = 0.0f, prevPhase = 0.0f, prevRaw = 0.0f, prevTheta = 0.0f;
+  for (int w = 0; w < c.freq; w++) {
+    if (!P.pok32[w]) { ok = false; break; }
+    C x[CE_MAX_STATESPACE_DIM];
+    if (!shifted_solve_t<float>(oa, ns, C{(float)P.z[w].r, (float)P.z[w].i}, L, x)) { ok = false; break; }
+    C G{0.0f, 0.0f};
+    for (int i = 0; i < ns; i++) G = C{G.r + K[i] * x[i].r, G.i + K[i] * x[i].i};
+    const C pr = P.presp32[w];
+    const C resp = cmul(pr, G);
+    const float m = hypotf_(resp.r, resp.i);
+    const float raw = cedm::atan2f_(resp.i, resp.r);
+    const float theta_w = (float)P.theta[w];
+    {
+      const float Ps = alpha * hypotf_(pr.r, pr.i), Cs = hypotf_(G.r, G.i) / alpha;
+      B = fminf_(B, hypotf_(1.0f + resp.r, resp.i) / (sqrtf_(1.0f + Ps * Ps) * sqrtf_(1.0f + Cs * Cs)));
+    }
+    if (w > 0) {
+      float delta = raw - prevRaw;
+      if (delta > PIF) delta = delta - 2.0f * PIF;
+      else if (delta < -PIF) delta = delta + 2.0f * PIF;
+      const float ph = prevPhase + delta;
+      if ((prevMag - 1.0f) * (m - 1.0f) <= 0.0f && m != prevMag) {
+        const float frac = (1.0f - prevMag) / (m - prevMag);
+        const float phase_c = prevPhase + frac * (ph - prevPhase);
+        crossed = true;
+        float pm_deg = 180.0f + phase_c * (180.0f / PIF);
+        pm_deg = pm_deg - 360.0f * floorf_((pm_deg + 180.0f) / 360.0f);
+        PM = fminf_(PM, pm_deg);
+        const float theta_c = prevTheta + frac * (theta_w - prevTheta);
+        const float pm_rad = pm_deg * (PIF / 180.0f);
+        if (theta_c > EPS6F && pm_rad >= 0.0f) DM = fminf_(DM, ((pm_rad / theta_c) * ts) * 1000.0f);
+      }
+      {
+        const float a = prevPhase + PIF, b = ph + PIF;
+        const float ka = floorf_(a / (2.0f * PIF)), kb = floorf_(b / (2.0f * PIF));
+        if (ka != kb && b != a) {
+          const float target = (2.0f * PIF) * (b > a ? kb : ka);
+          const float frac = (target - a) / (b - a);
+          const float mag_p = prevMag + frac * (m - prevMag);
+          GM = fminf_(GM, -20.0f * log10f_(mag_p));
+        }
+      }
+      prevPhase = ph;
+    } else {
+      prevPhase = raw;
+    }
+    prevMag = m;
+    prevRaw = raw;
+    prevTheta = theta_w;
+  }
+  if (!ok) {
+    sc[1] = sc[0]; sc[2] = EF; sc[0] = EF;
+    am[0] = 0.0f; am[1] = 0.0f; am[2] = 0.0f; am[3] = 1.0f; am[4] = 0.0f;
+    return;
+  }
+  const float err_gm = fmaxf_(0.0f, c.target_gm - GM) / c.target_gm;
+  const float err_pm = crossed ? fmaxf_(0.0f, c.target_pm - PM) / c.target_pm : 0.0f;
+  const float err_b = fmaxf_(0.0f, c.target_b - B) / c.target_b;
+  const float margin = (err_gm * err_gm + err_pm * err_pm) + err_b * err_b;
+  sc[1] = sc[0];
+  sc[2] = margin;
+  am[0] = GM; am[1] = PM; am[2] = DM; am[3] = 1.0f; am[4] = B;
+  const float total = cedm::logf_((sc[0] + margin) + EPS30F);
+  sc[0] = finite(total) ? total : EF;
+}
+}  // namespace r32
+
 // discrete_solve<Kalman> (src/discrete_solves.cu:83-185), never launched by the reference, restated in
 // double: gains row [K (ns), N, R, Q (ns)] of 2 ns + 2 floats (the reference sizes the row 2 ns + 1,
 // src/config.cu:51, and reads one float past it).  Per step: y = DC(xp); u = U(N r - K xh); predict
@@ -1251,6 +1692,50 @@ ORA_EXPORT int ora_evaluate(const ce_config* cfg, const ce_plant* plant, const f
         for (int k = 0; k < iw; k++) info64[(size_t)t * iw + k] = row[k];
       if (info_out)
         for (int k = 0; k < iw; k++) info_out[(size_t)t * iw + k] = (float)row[k];
+    }
+  }
+  return 0;
+}
+
+// The float forms of the fit solve and the margin sweep's complex solve (ref-FP32 chain).
+ORA_EXPORT void ora_ldlt_solve32(const float* m, int n, const float* b, float* x) {
+  float M[4 * CE_MAX_TOTAL_DIM * CE_MAX_TOTAL_DIM];
+  std::memcpy(M, m, sizeof(float) * n * n);
+  std::memcpy(x, b, sizeof(float) * n);
+  ldlt_solve_t<float>(M, n, x);
+}
+
+ORA_EXPORT int ora_shifted_solve32(const float* a, int n, float zr, float zi, const float* b, float* xr, float* xi) {
+  CxT<float> x[CE_MAX_STATESPACE_DIM];
+  const bool ok = shifted_solve_t<float>(a, n, CxT<float>{zr, zi}, b, x);
+  for (int i = 0; i < n; i++) { xr[i] = x[i].r; xi[i] = x[i].i; }
+  return ok ? 1 : 0;
+}
+
+// ref-FP32 chain (the reference's precision map) for n candidates of one round: float INFO rows
+// (info_out, n x info_w) and, when y_out is not NULL, the float trajectories (n x time).
+ORA_EXPORT int ora_evaluate_ref32(const ce_config* cfg, const ce_plant* plant, const float* gains, const float* info_in,
+                                  float* info_out, float* y_out, int n, int threads, char* err, int errlen) {
+  Problem P;
+  std::string e;
+  if (!setup(*cfg, *plant, P, e)) { set_err(err, errlen, e); return 1; }
+  if (!P.alpha32_ok) {
+    set_err(err, errlen, "allmargin: plant has no finite nonzero float gain at the crossover");
+    return 1;
+  }
+  const int iw = P.d.info_w, gw = P.d.cuda_w;
+#if defined(_OPENMP)
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<float> y(cfg->time);
+#pragma omp for schedule(dynamic, 8)
+    for (int t = 0; t < n; t++) {
+      float* row = info_out + (size_t)t * iw;
+      for (int k = 0; k < iw; k++) row[k] = info_in[(size_t)t * iw + k];
+      r32::candidate(P, gains + (size_t)t * gw, row, y.data());
+      if (y_out) std::memcpy(y_out + (size_t)t * cfg->time, y.data(), sizeof(float) * cfg->time);
     }
   }
   return 0;

@@ -41,6 +41,13 @@ def lib():
         _lib.ora_max_threads.restype = ctypes.c_int
         _lib.ora_simulate_kalman.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_char_p, ctypes.c_int]
         _lib.ora_simulate_kalman.restype = ctypes.c_int
+        _lib.ora_evaluate_ref32.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                            ctypes.c_int]
+        _lib.ora_evaluate_ref32.restype = ctypes.c_int
+        _lib.ora_ldlt_solve32.argtypes = [vp, ctypes.c_int, vp, vp]
+        _lib.ora_ldlt_solve32.restype = None
+        _lib.ora_shifted_solve32.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp, vp, vp]
+        _lib.ora_shifted_solve32.restype = ctypes.c_int
     return _lib
 
 
@@ -70,6 +77,25 @@ def evaluate(problem, rnd=0, gains=None, info=None, threads=0, diag=False):
     if rc != 0:
         raise ValueError(err.value.decode())
     return out32, out64, dg
+
+
+def evaluate_ref32(problem, rnd=0, gains=None, info=None, threads=0, want_y=False):
+    """The reference's own precision map (float simulation with hi/lo state, float eigensetup, double
+    eigensolve stored as float, float modal fit and LDLT, complex<float> margin sweep) over the
+    candidate rows of one round.  Returns (info32, y32 or None)."""
+    cfg = problem.cfg_struct()
+    plant = problem.plant_struct(rnd)
+    g = np.ascontiguousarray(problem.gains if gains is None else gains, dtype=np.float32)
+    inf = np.ascontiguousarray(problem.info if info is None else info, dtype=np.float32)
+    n = g.shape[0]
+    out = np.zeros_like(inf)
+    y = np.zeros((n, cfg.time), dtype=np.float32) if want_y else None
+    err = ctypes.create_string_buffer(512)
+    rc = lib().ora_evaluate_ref32(ctypes.byref(cfg), ctypes.byref(plant), _p(g), _p(inf), _p(out), _p(y), n,
+                                  int(threads), err, len(err))
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return out, y
 
 
 def simulate_kalman(problem, gains, rnd=0):
@@ -122,6 +148,24 @@ def shifted_solve(a, z, b):
     xi = np.zeros(n)
     ok = lib().ora_shifted_solve(_p(a), n, float(np.real(z)), float(np.imag(z)), _p(b), _p(xr), _p(xi))
     return bool(ok), xr + 1j * xi
+
+
+def ldlt_solve32(m, b):
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    x = np.zeros_like(b)
+    lib().ora_ldlt_solve32(_p(m), m.shape[0], _p(b), _p(x))
+    return x
+
+
+def shifted_solve32(a, z, b):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    n = a.shape[0]
+    xr = np.zeros(n, dtype=np.float32)
+    xi = np.zeros(n, dtype=np.float32)
+    ok = lib().ora_shifted_solve32(_p(a), n, float(np.real(z)), float(np.imag(z)), _p(b), _p(xr), _p(xi))
+    return bool(ok), xr.astype(np.float64) + 1j * xi.astype(np.float64)
 
 
 def detmath(fn, x, y=None):

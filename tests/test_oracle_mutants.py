@@ -17,7 +17,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "ce_oracle.cpp")
 PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_pipeline.py", "tests/test_oracle_linalg.py",
-             "tests/test_oracle_kalman.py"]
+             "tests/test_oracle_kalman.py", "tests/test_oracle_ref32.py"]
 
 # (cell, text in the oracle, the same text with the mistake)
 MUTANTS = [
@@ -37,7 +37,8 @@ MUTANTS = [
      "fr[2] = peak / cedm::fmax_(yss, EPS6) - 1.0;"),
     ("overshoot peak", "peak = cedm::fmax_(peak, y[t] * yss);", "peak = cedm::fmax_(peak, y[t]);"),
     ("wn aggregate", "else { wn = cedm::exp(lw / esum); zeta = zw / esum; }", "else { wn = lw / esum; zeta = zw / esum; }"),
-    ("zeta weight", "zw = zw + energy * (-re_e / wn_e);", "zw = zw + (-re_e / wn_e);"),
+    ("zeta weight", "lw = lw + energy * cedm::log(wn_e);\n        zw = zw + energy * (-re_e / wn_e);",
+     "lw = lw + energy * cedm::log(wn_e);\n        zw = zw + (-re_e / wn_e);"),
     ("oscillatory filter", "if (im_e < EPS6) continue;", "if (im_e < -1.0) continue;"),
     ("zeta term", "double beh = lr * lr + ((zeta - dmp) * (zeta - dmp) / dmp) / dmp;",
      "double beh = lr * lr + ((zeta - dmp) * (zeta - dmp) / dmp);"),
@@ -46,7 +47,8 @@ MUTANTS = [
      "const double err_os = cedm::fmax_(0.0, fr[2]) / os;"),
     ("steady-state hinge", "cedm::fabs_(fr[3] / (double)c.ref_amplitude - 1.0) - sse", "cedm::fabs_(fr[3] - 1.0) - sse"),
     ("nominal loop sign", "Acl[i * m + ns + j] = -(L[i] * P.pc[j]);", "Acl[i * m + ns + j] = (L[i] * P.pc[j]);"),
-    ("nominal-unstable margin cell", "sc[2] = UE;\n    am[0]", "sc[2] = sc[0];\n    am[0]"),
+    ("nominal-unstable margin cell", "sc[2] = UE;\n    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;",
+     "sc[2] = sc[0];\n    am[0] = am[1] = am[2] = am[3] = am[4] = 0.0;"),
     ("nominal-unstable total", "const double total = cedm::log((sc[0] + UE) + EPS30);",
      "const double total = cedm::log(sc[0] + EPS30);"),
     ("b denominator", "B = cedm::fmin_(B, num / (cedm::sqrt_(1.0 + Ps * Ps) * cedm::sqrt_(1.0 + Cs * Cs)));",
@@ -77,6 +79,25 @@ MUTANTS = [
     ("kalman process noise", "Pn[i * ns + i] = Pn[i * ns + i] + Q[i];", "Pn[i * ns + i] = Pn[i * ns + i];"),
     ("kalman joseph term", "Pm[i * ns + j] = a + (kg[i] * R) * kg[j];", "Pm[i * ns + j] = a;"),
     ("sim observer innovation", "const double yerr = yv - yhat;", "const double yerr = yhat - yv;"),
+    ("ref32 TwoSum error term", "const float v = (LO + l) + sl;", "const float v = LO + l;"),
+    ("ref32 plant state uncompensated", "for (int i = 0; i < nd; i++) hladd(xp[i], xpl[i], ph[i], pl[i]);",
+     "for (int i = 0; i < nd; i++) xp[i] = xp[i] + ph[i];"),
+    ("ref32 lina B K sign", "lina[i * nt + j] = ((float)P.sa[i * ns + j] - bk) - lcd;",
+     "lina[i * nt + j] = ((float)P.sa[i * ns + j] + bk) - lcd;"),
+    ("ref32 lina -BU K sign", "lina[(ns + i) * nt + j] = -((float)P.dbu[i] * K[j]);",
+     "lina[(ns + i) * nt + j] = ((float)P.dbu[i] * K[j]);"),
+    ("ref32 overshoot normalisation", "fr[2] = peak / fmaxf_(yss * yss, EPS6F) - 1.0f;",
+     "fr[2] = peak / fmaxf_(yss, EPS6F) - 1.0f;"),
+    ("ref32 energy cross term", "((c0 * c0) * g00 + ((2.0f * c0) * c1) * g01) + (c1 * c1) * g11;",
+     "((c0 * c0) * g00 + (c0 * c1) * g01) + (c1 * c1) * g11;"),
+    ("ref32 GM sign", "GM = fminf_(GM, -20.0f * log10f_(mag_p));", "GM = fminf_(GM, 20.0f * log10f_(mag_p));"),
+    ("ref32 PM offset", "float pm_deg = 180.0f + phase_c * (180.0f / PIF);", "float pm_deg = phase_c * (180.0f / PIF);"),
+    ("ref32 b plant scaling", "const float Ps = alpha * hypotf_(pr.r, pr.i)", "const float Ps = hypotf_(pr.r, pr.i)"),
+    ("ref32 nominal loop sign", "Acl[i * m + ns + j] = -(L[i] * pc[j]);", "Acl[i * m + ns + j] = (L[i] * pc[j]);"),
+    ("ref32 margin total", "const float total = cedm::logf_((sc[0] + margin) + EPS30F);",
+     "const float total = cedm::logf_(sc[0] + EPS30F);"),
+    ("ref32 plant response", "acc = CxT<float>{acc.r + pc[i] * x[i].r, acc.i + pc[i] * x[i].i};",
+     "acc = CxT<float>{acc.r + pc[i] * x[i].r, acc.i - pc[i] * x[i].i};"),
 ]
 
 
