@@ -92,7 +92,8 @@ def host_info():
 
 
 DRIFT_KEYS = ("max_dgm_db", "max_dpm_deg", "max_db", "max_dovershoot", "max_rel_dwn", "max_rel_dyss", "n_margin_rows",
-              "passfail_mismatch", "pass_rank_identical", "stable_mismatch", "behaviour_rel_err_median",
+              "passfail_mismatch", "pass_rank_identical", "n_pass", "margin_pass_mismatch", "n_margin_pass",
+              "margin_pass_rank_identical", "margin_pass_rank_swaps", "stable_mismatch", "behaviour_rel_err_median",
               "behaviour_rel_err_p99", "score_rank_spearman")
 
 

@@ -29,6 +29,13 @@ def pass_flags(problem, info):
     return (sc0 < c["ue"]) & swept & (margin == 0.0) & os_ok & ss_ok
 
 
+def margin_pass_flags(problem, info):
+    """The margin part of the pass rule alone: stable, swept, GM, PM and b at or above their targets
+    (the cells the mixed chain's FP32 sweep decides)."""
+    c = columns(problem)
+    return (info[:, c["sc"]] < c["ue"]) & (info[:, c["am"] + 3] == 1.0) & (info[:, c["sc"] + 2] == 0.0)
+
+
 def ranking(problem, info):
     """Candidate indices by ascending score, ties by index; rows whose score is not a number last."""
     s = info[:, columns(problem)["sc"]]
@@ -67,26 +74,35 @@ def compare(problem, ref, got):
     top = max(1, nfin // 100)
     order_ref = np.argsort(s_ref)[:top]
     order_got = np.argsort(s_got)[:top]
-    # the ranked list of passing candidates: same members in the same order?
     rk_ref, rk_got = ranking(problem, ref), ranking(problem, got)
-    pass_rank_ref = rk_ref[pf_ref[rk_ref]]
-    pass_rank_got = rk_got[pf_got[rk_got]]
-    # score gaps (FP64) of adjacent passing candidates whose order differs
-    swaps = []
-    if np.array_equal(np.sort(pass_rank_ref), np.sort(pass_rank_got)):
-        pos = np.empty(ref.shape[0], dtype=np.int64)
-        pos[pass_rank_got] = np.arange(pass_rank_got.size)
-        for a, b in zip(pass_rank_ref[:-1], pass_rank_ref[1:]):
-            if pos[a] > pos[b]:
-                swaps.append(float(abs(s_ref[b] - s_ref[a]) / max(abs(s_ref[a]), 1e-300)))
+
+    def ranked(flags_ref, flags_got):
+        """The ranked lists of the flagged candidates: same members in the same order?  With the
+        FP64 score gaps of adjacent members whose order differs."""
+        a_ref, a_got = rk_ref[flags_ref[rk_ref]], rk_got[flags_got[rk_got]]
+        swaps = []
+        if np.array_equal(np.sort(a_ref), np.sort(a_got)):
+            pos = np.empty(ref.shape[0], dtype=np.int64)
+            pos[a_got] = np.arange(a_got.size)
+            for a, b in zip(a_ref[:-1], a_ref[1:]):
+                if pos[a] > pos[b]:
+                    swaps.append(float(abs(s_ref[b] - s_ref[a]) / max(abs(s_ref[a]), 1e-300)))
+        return bool(np.array_equal(a_ref, a_got)), swaps
+
+    pass_identical, swaps = ranked(pf_ref, pf_got)
+    mp_ref, mp_got = margin_pass_flags(problem, ref), margin_pass_flags(problem, got)
+    mp_identical, mp_swaps = ranked(mp_ref, mp_got)
     return dict(
         n=int(ref.shape[0]), n_margin_rows=int(both.sum()),
         max_dgm_db=dmax(am + 0), max_dpm_deg=dmax(am + 1), max_db=dmax(am + 4), max_ddm_ms=dmax(am + 2),
         max_dovershoot=dmax(fr + 2, fitted), max_rel_dwn=dmax(fr + 0, fitted, rel=True),
         max_rel_dyss=dmax(fr + 3, fitted, rel=True),
         passfail_mismatch=int(np.sum(pf_ref != pf_got)), n_pass=int(pf_ref.sum()),
-        pass_rank_identical=bool(np.array_equal(pass_rank_ref, pass_rank_got)),
+        pass_rank_identical=pass_identical,
         pass_rank_swaps=len(swaps), pass_rank_swap_max_rel_gap=max(swaps) if swaps else 0.0,
+        margin_pass_mismatch=int(np.sum(mp_ref != mp_got)), n_margin_pass=int(mp_ref.sum()),
+        margin_pass_rank_identical=mp_identical, margin_pass_rank_swaps=len(mp_swaps),
+        margin_pass_rank_swap_max_rel_gap=max(mp_swaps) if mp_swaps else 0.0,
         stable_mismatch=int(np.sum(stable_ref != stable_got)),
         behaviour_rel_err_median=float(np.median(rel_beh)) if rel_beh.size else 0.0,
         behaviour_rel_err_p99=float(np.quantile(rel_beh, 0.99)) if rel_beh.size else 0.0,

@@ -46,6 +46,9 @@ def test_full_block(engine, name, sample):
     assert rep["max_db"] <= 1e-4
     assert rep["passfail_mismatch"] == 0
     assert rep["stable_mismatch"] == 0
+    # the margin part of the pass rule alone (the full rule passes few or no candidates of these
+    # workloads): the same candidates meet GM, PM and b in both chains
+    assert rep["margin_pass_mismatch"] == 0
 
     ok = same_rows(m1, m2)
     assert ok.all(), "two mixed evaluations differ in %d cells" % int((~ok).sum())

@@ -52,6 +52,9 @@ def test_mixed_margins_match_fp64(engine, name, n):
     assert rep["max_db"] <= 1e-4
     assert rep["passfail_mismatch"] == 0
     assert rep["stable_mismatch"] == 0
+    # the margin part of the pass rule alone (the full rule passes few or no candidates of these
+    # workloads): the same candidates meet GM, PM and b in both chains
+    assert rep["margin_pass_mismatch"] == 0
     # behaviour score from the float trajectory: close wherever the least-squares fit is well posed
     kappa = fit_condition(p, n)
     d = p.derived()
