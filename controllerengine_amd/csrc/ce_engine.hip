@@ -260,7 +260,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // the specialised simulation kernel (compiled once per problem and form); with the second stream
   // and a grid that fits the device's SIMDs at one wave each, the gated form
   const int sim_grid = (n + 63) / 64;
-  if (fork && !E->n_simd) {
+  if (!E->n_simd) {
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device));
     E->n_simd = 4 * cus;
@@ -344,14 +344,19 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   if (fn) {
     const bool st_form = E->sim_fn_struct[mixed ? 1 : 0][gated ? 1 : 0];
     bool ok;
+    // lanes per candidate the float literal kernel may use: a grid of at most two waves per SIMD
+    // (the kernel picks the lanes from the live count, keeping its working waves one per SIMD)
+    int gq = 1;
+    if (mixed && !st_form) gq = (4 * sim_grid <= 2 * E->n_simd) ? 4 : ((2 * sim_grid <= 2 * E->n_simd) ? 2 : 1);
     if (st_form)
       ok = gated ? jit_launch_sim_struct_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
                                                n, (int*)E->gate.p, E->gate_gen, P)
                  : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P);
     else
-      ok = gated ? jit_launch_sim_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
-                                        (int*)E->gate.p, E->gate_gen)
-                 : jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n);
+      ok = gated ? jit_launch_sim_gated(fn, gq * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
+                                        (int*)E->gate.p, E->gate_gen, gq, E->n_simd)
+                 : jit_launch_sim(fn, gq * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, gq,
+                                  E->n_simd);
     if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");
     E->last_sim_name = st_form ? (mixed ? (gated ? "ce_sims_f32_g" : "ce_sims_f32") : (gated ? "ce_sims_f64_g" : "ce_sims_f64"))
                                : (mixed ? (gated ? "ce_sim_f32_g" : "ce_sim_f32") : (gated ? "ce_sim_f64_g" : "ce_sim_f64"));

@@ -119,6 +119,29 @@ struct Gen {
 
 }  // namespace
 
+// The time loop over the generated step: the horizon in segments between the phase boundaries,
+// within each of which the phase flags are compile-time constants.
+static void gen_phase_loop(std::ostringstream& o) {
+  o << "  const bool wdy = dy != nullptr;\n"
+       "  int t = 0;\n"
+       "  while (t < TT) {\n"
+       "    int e = TT;\n"
+       "    if (rs > t && rs < e) e = rs;\n"
+       "    if (f0 > t && f0 < e) e = f0;\n"
+       "    if (f1 > t && f1 < e) e = f1;\n"
+       "    if (s0 > t && s0 < e) e = s0;\n"
+       "    const bool st = t >= f0 && t < f1, ss = t >= s0, rz = t < rs;\n"
+       "    if (wdy) {\n"
+       "      for (; t < e; t++) step(t, st, ss, rz, true);\n"
+       "      continue;\n"
+       "    }\n"
+       "    switch ((st ? 4 : 0) | (ss ? 2 : 0) | (rz ? 1 : 0)) {\n";
+  for (int k = 0; k < 8; k++)
+    o << "      case " << k << ": for (; t < e; t++) step(t, BC<" << (k >> 2 & 1) << ">{}, BC<" << (k >> 1 & 1) << ">{}, BC<"
+      << (k & 1) << ">{}, BC<0>{}); break;\n";
+  o << "    }\n  }\n";
+}
+
 // Per-candidate body shared by both forms.  Literal form: every constant and size is written into
 // the source.  Structural form: only the dimensions and the instruction structure (opcodes,
 // columns, operand kinds) are fixed; constants and sizes are loaded from the problem descriptor
@@ -310,25 +333,7 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
     o.seekp(0, std::ios_base::end);
   }
   if (n_hoisted) *n_hoisted = hwords;
-  // the horizon in segments between the phase boundaries; within one the flags are constant
-  o << "  const bool wdy = dy != nullptr;\n"
-       "  int t = 0;\n"
-       "  while (t < TT) {\n"
-       "    int e = TT;\n"
-       "    if (rs > t && rs < e) e = rs;\n"
-       "    if (f0 > t && f0 < e) e = f0;\n"
-       "    if (f1 > t && f1 < e) e = f1;\n"
-       "    if (s0 > t && s0 < e) e = s0;\n"
-       "    const bool st = t >= f0 && t < f1, ss = t >= s0, rz = t < rs;\n"
-       "    if (wdy) {\n"
-       "      for (; t < e; t++) step(t, st, ss, rz, true);\n"
-       "      continue;\n"
-       "    }\n"
-       "    switch ((st ? 4 : 0) | (ss ? 2 : 0) | (rz ? 1 : 0)) {\n";
-  for (int k = 0; k < 8; k++)
-    o << "      case " << k << ": for (; t < e; t++) step(t, BC<" << (k >> 2 & 1) << ">{}, BC<" << (k >> 1 & 1) << ">{}, BC<"
-      << (k & 1) << ">{}, BC<0>{}); break;\n";
-  o << "    }\n  }\n";
+  gen_phase_loop(o);
   o << "  if (unstable) return;\n";
   // yss and overshoot (reference src/modalsetup.cu:52-62)
   o << "  const " << T << " yss = ysum / (" << T << ")(TT - s0);\n";
@@ -340,6 +345,149 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
     o << "  fr[2] = (double)(pk / __builtin_fmaxf(yss * yss, " << litf((float)EPS6) << ") - 1.0f);\n  fr[3] = (double)yss;\n";
   }
 #undef CE_OFF
+}
+
+// Lane-group form of the float literal kernel: G lanes (2 or 4) per candidate, for blocks whose
+// live candidates leave SIMDs idle at one lane each.  Every lane of a group runs the instruction
+// chains, K x and C x over the whole state, as the one-lane form does; lane j updates the rows
+// G s + j of the observer and plant state (the same terms in the same order, then the compensated
+// add), and each group's new rows are broadcast to its lanes with DPP quad permutes.  Every row
+// therefore sees the one-lane form's operations: the trajectory is bit-identical.  A lane's row
+// coefficients are selected once, before the time loop.
+static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
+  const int ns = P.ns, nd = P.nd, nw = P.nw;
+  const int lg = G == 4 ? 2 : 1;
+  std::ostringstream& o = g.o;
+  std::ostringstream pre;
+  int nsel = 0;
+  auto all_zero = [](const std::vector<double>& v) {
+    for (double x : v)
+      if ((float)x != 0.0f) return false;
+    return true;
+  };
+  // this lane's value of a per-row coefficient (v[j]: row G s + j; absent rows 0)
+  auto sel = [&](const std::vector<double>& v) -> std::string {
+    bool same = true;
+    for (int j = 1; j < G; j++)
+      if ((float)v[j] != (float)v[0]) same = false;
+    if (same) return g.L(v[0]);
+    const std::string nm = "pc" + std::to_string(nsel++);
+    pre << "  const float " << nm << " = ";
+    for (int j = G - 1; j >= 1; j--) pre << "h == " << j << " ? " << g.L(v[j]) << " : ";
+    pre << g.L(v[0]) << ";\n";
+    return nm;
+  };
+  auto rowv = [&](int s, int n, auto f) {
+    std::vector<double> v(G, 0.0);
+    for (int j = 0; j < G; j++)
+      if (G * s + j < n) v[j] = f(G * s + j);
+    return v;
+  };
+  // DPP quad permute that gives every lane of a group the value of the group's lane j
+  auto bc = [&](int j) {
+    int p[4];
+    for (int q = 0; q < 4; q++) p[q] = (q / G) * G + j;
+    return "ce_bc<" + std::to_string(p[0] | p[1] << 2 | p[2] << 4 | p[3] << 6) + ">";
+  };
+  const int S = (ns + G - 1) / G, SP = (nd + G - 1) / G;
+  o << "  const int h = threadIdx.x & " << G - 1 << ";\n";
+  o << "  const int idx = bx * " << 64 / G << " + (threadIdx.x >> " << lg << ");\n  if (idx >= *A.count) return;\n"
+       "  const int c = A.list[idx];\n";
+  o << "  const int n = A.n;\n  const int TT = " << P.T << ", f0 = " << P.fit_start_id << ", f1 = "
+    << P.fit_start_id + P.fit_count << ", s0 = " << P.steady_start_id << ", rs = " << P.ref_step_id << ";\n";
+  o << "  const bool unstable = (A.flags[c] & 1) != 0;\n";
+  o << "  const float* g = A.gains + (long)c * " << P.cuda_w << ";\n";
+  for (int i = 0; i < ns; i++) o << "  const float K" << i << " = g[" << i << "];\n";
+  for (int s = 0; s < S; s++)
+    o << "  const float Lo" << s << " = (" << G * s << " + h < " << ns << ") ? g[" << ns + G * s << " + h] : 0.0f;\n";
+  o << "  const float N = g[" << 2 * ns << "];\n";
+  for (int i = 0; i < ns; i++) o << "  float xh" << i << " = 0.0f, xhl" << i << " = 0.0f;\n";
+  for (int i = 0; i < nd; i++) o << "  float xp" << i << " = 0.0f, xpl" << i << " = 0.0f;\n";
+  for (int s = 0; s < S; s++) o << "  float oh" << s << " = 0.0f, ol" << s << " = 0.0f;\n";
+  for (int s = 0; s < SP; s++) o << "  float op" << s << " = 0.0f, opl" << s << " = 0.0f;\n";
+  o << "  float ysum = 0.0f, ymax = -__builtin_inff(), ymin = __builtin_inff();\n";
+  o << "  float* yb = (float*)A.ybuf + idx;\n  float* dy = A.dy;\n";
+  o << "  const float amp = " << g.L(P.ref_amp) << ";\n";
+  const std::string mark = "  /*lane rows*/\n";
+  o << mark;
+  o << "  auto step = [&](const int t, auto ST, auto SS, auto RZ, auto DY) __attribute__((always_inline)) {\n";
+  std::vector<std::string> sxp, sxh;
+  for (int i = 0; i < nd; i++) sxp.push_back("xp" + std::to_string(i));
+  for (int i = 0; i < ns; i++) sxh.push_back("xh" + std::to_string(i));
+  auto litc = [&](int, int, double v) { return g.L(v); };
+  const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "", litc);
+  o << "    const float y = " << y << ";\n";
+  o << "    if (DY) dy[(long)c * TT + t] = y;\n";
+  o << "    if (ST) yb[(long)(t - f0) * n] = y;\n";
+  o << "    if (SS) ysum = ysum + y;\n";
+  o << "    ymax = ce_maxf(ymax, y); ymin = ce_minf(ymin, y);\n";
+  o << "    float kx = 0.0f;\n";
+  for (int k = 0; k < ns; k++) o << "    kx = __builtin_fmaf(K" << k << ", xh" << k << ", kx);\n";
+  for (int k = 0; k < ns; k++) o << "    kx = __builtin_fmaf(K" << k << ", xhl" << k << ", kx);\n";
+  o << "    const float rr = RZ ? 0.0f : amp;\n";
+  o << "    const float u_raw = __builtin_fmaf(N, rr, -kx);\n";
+  const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", litc);
+  o << "    const float uu = " << u << ";\n";
+  o << "    float cx = 0.0f;\n";
+  for (int k = 0; k < ns; k++)
+    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sc[k]) << ", xh" << k << ", cx);\n";
+  for (int k = 0; k < ns; k++)
+    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sc[k]) << ", xhl" << k << ", cx);\n";
+  if ((float)P.sd != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sd) << ", uu, cx);\n";
+  o << "    const float yerr = y - cx;\n";
+  for (int s = 0; s < S; s++) {
+    const auto vb = rowv(s, ns, [&](int r) { return P.sb[r]; });
+    if (all_zero(vb)) o << "    float dh" << s << " = Lo" << s << " * yerr;\n";
+    else o << "    float dh" << s << " = __builtin_fmaf(Lo" << s << ", yerr, " << sel(vb) << " * uu);\n";
+    o << "    float dl" << s << " = 0.0f;\n";
+    for (int k = 0; k < ns; k++) {
+      const auto va = rowv(s, ns, [&](int r) { return P.sai[r * ns + k]; });
+      if (all_zero(va)) continue;
+      const std::string cf = sel(va);
+      o << "    dh" << s << " = __builtin_fmaf(" << cf << ", xh" << k << ", dh" << s << ");\n";
+      o << "    dl" << s << " = __builtin_fmaf(" << cf << ", xhl" << k << ", dl" << s << ");\n";
+    }
+  }
+  for (int s = 0; s < S; s++) o << "    ce_hladd(oh" << s << ", ol" << s << ", dh" << s << ", dl" << s << ");\n";
+  for (int s = 0; s < S; s++)
+    for (int j = 0; j < G && G * s + j < ns; j++) {
+      const int r = G * s + j;
+      o << "    xh" << r << " = " << bc(j) << "(oh" << s << "); xhl" << r << " = " << bc(j) << "(ol" << s << ");\n";
+    }
+  std::vector<std::string> snl = sxp;
+  snl.push_back("y");
+  std::vector<std::string> W;
+  for (int q = 0; q < nw; q++) W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, "", litc));
+  for (int s = 0; s < SP; s++) {
+    o << "    float ph" << s << " = 0.0f, pl" << s << " = 0.0f;\n";
+    auto term = [&](const char* acc, const std::vector<double>& v, const std::string& x) {
+      if (!all_zero(v)) o << "    " << acc << s << " = __builtin_fmaf(" << sel(v) << ", " << x << ", " << acc << s << ");\n";
+    };
+    term("ph", rowv(s, nd, [&](int r) { return P.dbu[r]; }), "uu");
+    for (int q = 0; q < nw; q++) term("ph", rowv(s, nd, [&](int r) { return P.dbw[r * nw + q]; }), W[q]);
+    for (int k = 0; k < nd; k++) term("ph", rowv(s, nd, [&](int r) { return P.dai[r * nd + k]; }), "xp" + std::to_string(k));
+    for (int k = 0; k < nd; k++) term("pl", rowv(s, nd, [&](int r) { return P.dai[r * nd + k]; }), "xpl" + std::to_string(k));
+  }
+  for (int s = 0; s < SP; s++) o << "    ce_hladd(op" << s << ", opl" << s << ", ph" << s << ", pl" << s << ");\n";
+  for (int s = 0; s < SP; s++)
+    for (int j = 0; j < G && G * s + j < nd; j++) {
+      const int r = G * s + j;
+      o << "    xp" << r << " = " << bc(j) << "(op" << s << "); xpl" << r << " = " << bc(j) << "(opl" << s << ");\n";
+    }
+  o << "  };\n";
+  {
+    std::string text = o.str();
+    const size_t at = text.find(mark);
+    text.replace(at, mark.size(), pre.str());
+    o.str(text);
+    o.seekp(0, std::ios_base::end);
+  }
+  gen_phase_loop(o);
+  o << "  if (unstable || h != 0) return;\n";
+  o << "  const float yss = ysum / (float)(TT - s0);\n";
+  o << "  const float pk = ce_maxf(0.0f, ((yss < 0.0f) ? ymin : ymax) * yss);\n";
+  o << "  double* fr = A.info + (long)c * " << P.info_w << " + " << P.info_freqresp << ";\n";
+  o << "  fr[2] = (double)(pk / __builtin_fmaxf(yss * yss, " << litf((float)EPS6) << ") - 1.0f);\n  fr[3] = (double)yss;\n";
 }
 
 static const char* kSimArgs =
@@ -356,7 +504,10 @@ static const char* kSimArgs =
     // compensated state update (HI, LO) += (h, l): TwoSum, then Fast2Sum
     "static __device__ inline __attribute__((always_inline)) void ce_hladd(float& H, float& Lo, float h, float l) {\n"
     "  const float sh = H + h; const float b0 = sh - H; const float sl = (H - (sh - b0)) + (h - b0);\n"
-    "  const float v = (Lo + l) + sl; H = sh + v; Lo = v - (H - sh); }\n";
+    "  const float v = (Lo + l) + sl; H = sh + v; Lo = v - (H - sh); }\n"
+    // the value of lane CTRL-selected within each quad (DPP quad permute), for the lane-group form
+    "template <int CTRL> static __device__ inline __attribute__((always_inline)) float ce_bc(float v) {\n"
+    "  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false)); }\n";
 
 // Both forms raise the simulation waves' issue priority (s_setprio 3): the simulation is one serial
 // recurrence per candidate with a single wave on each SIMD it uses, and the kernels that run beside
@@ -389,6 +540,8 @@ static void gen_struct_body(std::ostringstream& o, const ProblemDev& P, bool mix
 // Single-problem kernel entry: the literal form (every constant in the source) or the structural
 // form (constants read from the problem descriptor passed as the last argument).
 static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool structural) {
+  // lane groups: the float literal kernel also carries 2- and 4-lane forms of the step
+  const bool grp = mixed && !structural;
   Gen g;
   g.f32 = mixed;
   std::ostringstream& o = g.o;
@@ -401,18 +554,33 @@ static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool 
   if (structural) gen_struct_body(o, P, mixed);
   else gen_body(g, P, mixed, false);
   o << "}\n";
+  if (grp) {
+    for (int G : {2, 4}) {
+      Gen gg;
+      gg.f32 = true;
+      gen_body_grp(gg, P, G);
+      o << "static __device__ inline __attribute__((always_inline)) void sim_g" << G
+        << "(const SimArgs& A, int bx) {\n" << gg.o.str() << "}\n";
+    }
+  }
   o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
     << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
        "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
        "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
-    << (structural ? ", const char* __restrict__ probp" : "") << ") {\n"
+    << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap") << ") {\n"
        "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
   if (gated)
     o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
          "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
          "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
-  o << "  __builtin_amdgcn_s_setprio(3);\n"
-    << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
+  o << "  __builtin_amdgcn_s_setprio(3);\n";
+  if (grp)
+    // lanes per candidate: the most the launch allows (gq) with the live candidates' waves still
+    // at most cap, one per SIMD
+    o << "  const int live = __builtin_amdgcn_readfirstlane(*count);\n"
+         "  if (gq >= 4 && live <= 16 * cap) { sim_g4(A, blockIdx.x); return; }\n"
+         "  if (gq >= 2 && live <= 32 * cap) { sim_g2(A, blockIdx.x); return; }\n";
+  o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
   return o.str();
 }
 
@@ -561,14 +729,15 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
 }
 
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, void* ybuf, double* info, float* dy, int n) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n};
+                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gq, &cap};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 
 bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gate, &gen};
+                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen, int gq,
+                          int cap) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gate, &gen, &gq, &cap};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

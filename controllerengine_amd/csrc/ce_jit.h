@@ -32,11 +32,15 @@ bool jit_launch_sim_struct(void* fn, int grid, hipStream_t st, const float* gain
 bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                                  const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen,
                                  const void* prob);
+// Literal form.  gq: lanes per candidate the launch allows (1, 2 or 4; the grid covers gq lanes per
+// candidate), cap: waves the device runs one per SIMD.  The float kernel runs the most lanes per
+// candidate, up to gq, whose live candidates' waves stay within cap; the FP64 kernel runs one.
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, void* ybuf, double* info, float* dy, int n);
+                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap);
 // the gated form: its last workgroup stores gen to *gate when it starts
 bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen);
+                          const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen, int gq,
+                          int cap);
 
 // Batched form: one kernel per instruction structure and dimensions (sim_structure_key), constants
 // and sizes read from each problem's descriptor.  SimArgs mirrors the struct in the generated

@@ -81,6 +81,25 @@ def test_mixed_specialised_and_interpreted_agree(engine, monkeypatch):
     assert same.all(), "float simulation kernels differ in %d cells" % int((~same).sum())
 
 
+@pytest.mark.parametrize("name,n,lanes", [("c2", 65536, 2), ("c1", 1024, 4), ("c3", 1024, 4)])
+def test_mixed_lane_groups_equal_interpreted(engine, name, n, lanes, monkeypatch):
+    """The float literal kernel splits a candidate's step over 2 or 4 lanes when the live
+    candidates leave SIMDs idle (c2's full block: ~27k simulated candidates, 2 lanes; 1,024
+    candidates: 4 lanes).  Its rows equal the interpreted one-lane kernel's bit for bit."""
+    p = workloads.CONFIGS[name](n=n)
+    print("case", name, n, "lanes", lanes, flush=True)
+    monkeypatch.setenv("CE_JIT", "1")
+    _, a = engine.evaluate(p, PRECISION_MIXED)
+    monkeypatch.setenv("CE_JIT", "0")
+    _, b = engine.evaluate(p, PRECISION_MIXED)
+    sc = p.derived().info_scoring
+    live = int((a[:, sc] < float(p.config["unstable_err"])).sum())
+    cap = 1024  # SIMDs of an MI355X
+    assert (live <= 16 * cap) == (lanes == 4) and (16 * cap < live <= 32 * cap) == (lanes == 2), live
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    assert same.all(), "lane-group and interpreted float kernels differ in %d cells" % int((~same).sum())
+
+
 @pytest.mark.parametrize("name,n", [("c2", 4096), ("c3", 1024)])
 def test_mixed_two_streams_equal_one_stream(engine, name, n):
     """The default mixed chain runs the nominal eigensolve, the sweep and the refinement on a second
