@@ -193,7 +193,7 @@ def bench_c4(args, steps=None, fp64_steps=None):
         dt = time.perf_counter() - t0
         info = b.info()
         res[name] = dict(value=n * rounds / dt, ms_per_round=1000 * dt / rounds, rounds=rounds,
-                         used_graph=info["used_graph"], launches=info["launches"],
+                         used_graph=info["used_graph"], launches=info["launches"], plan=b.plan(),
                          kernel_ms={k: round(v, 3) for k, v in kt})
     b.close()
     out = dict(workload="c4: %d plants of 2-8 states, %d candidates per round (log-uniform 256-16384 per plant), "

@@ -72,7 +72,7 @@ EXPORTS = [
     "ce_engine_last_sim_kernel", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_engine_sweep_counts",
     "ce_detmath_host",
     "ce_detmath_device", "ce_batch_create", "ce_batch_destroy", "ce_batch_clear", "ce_batch_add", "ce_batch_update",
-    "ce_batch_offsets", "ce_batch_set_options", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
+    "ce_batch_offsets", "ce_batch_set_options", "ce_batch_set_plan", "ce_batch_plan", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
     "ce_server_create", "ce_server_destroy", "ce_server_submit", "ce_server_poll", "ce_server_drain",
     "ce_server_stats", "ce_server_reset_stats", "ce_rank", "ce_rank_noalloc", "ce_simulate_kalman",
 ]
@@ -134,6 +134,10 @@ def _declare(lib):
     lib.ce_batch_create.restype = vp
     lib.ce_batch_destroy.argtypes = [vp]
     lib.ce_batch_destroy.restype = None
+    lib.ce_batch_set_plan.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.ce_batch_set_plan.restype = ctypes.c_int
+    lib.ce_batch_plan.argtypes = [vp, P(ctypes.c_int)]
+    lib.ce_batch_plan.restype = ctypes.c_int
     lib.ce_batch_clear.argtypes = [vp]
     lib.ce_batch_clear.restype = ctypes.c_int
     lib.ce_batch_add.argtypes = [vp, P(CeConfig), P(CePlant), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]

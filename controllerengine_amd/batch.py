@@ -71,6 +71,18 @@ class Batch:
     def set_options(self, use_graph=True, autotune=False, profiling=False):
         _abi.lib().ce_batch_set_options(self.h, int(use_graph), int(autotune), int(profiling))
 
+    def set_plan(self, eig_lin=-1, eig_nom=-1, sim=-1):
+        """Fix the stage plans (eigen stages: 0 per capacity, default lanes; 1 per capacity,
+        alternative lanes; 2 fused; simulation: 0 per structure, 1 fused); -1 hands them back to
+        the autotuner."""
+        if _abi.lib().ce_batch_set_plan(self.h, int(eig_lin), int(eig_nom), int(sim)) != 0:
+            raise ValueError("invalid batch plan")
+
+    def plan(self):
+        buf = (ctypes.c_int * 3)()
+        _abi.lib().ce_batch_plan(self.h, buf)
+        return dict(eig_lin=buf[0], eig_nom=buf[1], sim=buf[2])
+
     def info(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _abi.lib().ce_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))

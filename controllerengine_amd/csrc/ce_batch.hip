@@ -74,8 +74,10 @@ struct Launch {
   int key;            // template key
   std::string skey;   // simulation structure key
   std::vector<int> pid, start;
+  std::vector<int> sid;  // fused simulation: structure index of each problem (pid order)
   int32_t* d_start = nullptr;
   int32_t* d_pid = nullptr;
+  int32_t* d_sid = nullptr;
   void* fn = nullptr;  // simulation kernel
   int grid = 0;        // chosen grid
   std::string label;
@@ -136,6 +138,12 @@ struct ce_batch {
   // tuning
   int autotune = 0;
   bool tuned = false;
+  // stage plans: [0] LIN_A eigensolve, [1] nominal eigensolve: 0 one launch per matrix capacity
+  // with the default lanes per candidate, 1 the same with the alternative lanes, 2 one fused launch
+  // over every order; [2] simulation: 0 one launch per instruction structure, 1 one fused launch.
+  // The autotuner picks them unless a caller fixed them (ce_batch_set_plan).
+  int plan[3] = {0, 0, 0};
+  bool plan_fixed = false;
   // per-launch timing
   int profiling = 0;
   std::vector<hipEvent_t> mark_ev;
@@ -202,7 +210,6 @@ static void build_blobs(ce_batch* b, int precision) {
 }
 
 // eigen kernels: key = matrix capacity (4, 8, 16), candidates per workgroup = 64 / lanes
-static int eig_cpw(int m) { return 64 / eig_lanes(eig_key(m)); }
 
 // Compiled simulation kernels are shared by every batch of the process on the same device (the
 // caches are thread-safe); modules stay loaded until the process ends.
@@ -343,7 +350,7 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   }
   // launch groups, in chain order
   b->launches.clear();
-  auto group = [&](int kind, auto keyf, auto blocksf, const char* name) {
+  auto group = [&](int kind, auto keyf, auto blocksf, const char* name, auto labelf) {
     std::map<int, Launch> g;
     for (int j = 0; j < J; j++) {
       const int key = keyf(b->P[j]);
@@ -355,24 +362,64 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
       L.start.push_back(L.start.back() + blocksf(b->P[j], ps[j]));
     }
     for (auto& kv : g) {
-      kv.second.label = std::string(name) + (kv.first ? "/" + std::to_string(kv.first) : std::string());
+      kv.second.label = std::string(name) + labelf(kv.first);
       b->launches.push_back(kv.second);
     }
   };
+  auto num_label = [](int k) { return k ? "/" + std::to_string(k) : std::string(); };
   auto cdiv = [](int a, int c) { return (a + c - 1) / c; };
   using PP = ce_batch::Prob;
-  group(BK_EIG_LIN, [&](const PP& p) { return eig_key(p.nt); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(p.nt)); }, "k_eig_lin");
-  group(BK_COMPACT, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_compact");
+  // eigen stages: key = capacity * 100 + lanes per candidate (the plan's tile), 0 for the fused launch
+  auto eig_tile = [](int cap, int plan) {
+    if (plan == 2) return 0;
+    if (cap == 4) return 404;
+    if (cap == 8) return plan == 1 ? 808 : 800 + EIG8_G;
+    return plan == 1 ? 1608 : 1600 + EIG16_G;
+  };
+  auto eig_blocks = [&](int order, int plan, int n) {
+    const int cap = eig_key(order), key = eig_tile(cap, plan);
+    const int lanes = key ? key % 100 : eig_lanes(cap);
+    return cdiv(n, 64 / lanes);
+  };
+  auto eig_label = [](int k) { return k ? "/" + std::to_string(k / 100) + "x" + std::to_string(k % 100) : std::string("/all"); };
+  group(BK_EIG_LIN, [&](const PP& p) { return eig_tile(eig_key(p.nt), b->plan[0]); },
+        [&](const PP& p, const PS&) { return eig_blocks(p.nt, b->plan[0], p.n); }, "k_eig_lin", eig_label);
+  group(BK_COMPACT, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_compact", num_label);
   // the nominal eigensolve walks each problem's compacted list, so it follows the compaction
-  group(BK_EIG_NOM, [&](const PP& p) { return eig_key(2 * p.ns); }, [&](const PP& p, const PS&) { return cdiv(p.n, eig_cpw(2 * p.ns)); }, "k_eig_nom");
+  group(BK_EIG_NOM, [&](const PP& p) { return eig_tile(eig_key(2 * p.ns), b->plan[1]); },
+        [&](const PP& p, const PS&) { return eig_blocks(2 * p.ns, b->plan[1], p.n); }, "k_eig_nom", eig_label);
   {  // simulation: one group per structure, its problems longest horizon first (a wave's work is
      // proportional to the horizon: the long ones start first instead of forming the tail)
     std::vector<int> order(J);
     for (int j = 0; j < J; j++) order[j] = j;
     auto horizon = [&](int j) { return reinterpret_cast<const ProblemDev*>(b->P[j].blob.data())->T; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return horizon(x) > horizon(y); });
+    if (b->plan[2] == 1) {  // every structure in one launch
+      Launch L;
+      L.kind = SIM_LAUNCH;
+      L.key = 1;
+      L.start.push_back(0);
+      std::map<std::string, int> sidx;
+      std::vector<const ProblemDev*> reps;
+      for (const int j : order) {
+        const ProblemDev* H = reinterpret_cast<const ProblemDev*>(b->P[j].blob.data());
+        const std::string k = sim_structure_key(*H, mixed);
+        auto it = sidx.find(k);
+        if (it == sidx.end()) {
+          it = sidx.emplace(k, (int)reps.size()).first;
+          reps.push_back(H);
+        }
+        L.pid.push_back(j);
+        L.sid.push_back(it->second);
+        L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64));
+      }
+      std::string e;
+      if (!jit_get(shared_jit(b->device), gen_sim_batch_fused_source(reps, mixed), &L.fn, e)) throw std::runtime_error(e);
+      L.label = std::string(mixed ? "ce_simb_f32" : "ce_simb_f64") + "/all";
+      b->launches.push_back(L);
+    }
     std::map<std::string, Launch> g;
-    for (const int j : order) {
+    for (const int j : (b->plan[2] == 1 ? std::vector<int>() : order)) {
       const ProblemDev& H = *reinterpret_cast<const ProblemDev*>(b->P[j].blob.data());
       const std::string k = sim_structure_key(H, mixed);
       Launch& L = g[k];
@@ -395,20 +442,20 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   auto nb = [](const PP& p) { return (2 * p.nt + 7) / 8; };
   if (mixed) {
     group(BK_ATB, [](const PP&) { return 0; },
-          [&](const PP& p, const PS& s) { return cdiv(p.n, 64) * cdiv(p.nt, 8) * s.parts; }, "k_atb");
-    group(BK_FITCF, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fitcf");
-    group(BK_SWEEP_MARK32, [](const PP& p) { return p.ns; }, [](const PP& p, const PS&) { return p.n; }, "k_sweep_mark32");
-    group(BK_REFINE64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_refine64");
-    group(BK_FULL64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_full64");
-    group(BK_SCAN32, [](const PP&) { return 0; }, [](const PP& p, const PS&) { return p.n; }, "k_scan32");
+          [&](const PP& p, const PS& s) { return cdiv(p.n, 64) * cdiv(p.nt, 8) * s.parts; }, "k_atb", num_label);
+    group(BK_FITCF, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fitcf", num_label);
+    group(BK_SWEEP_MARK32, [](const PP& p) { return p.ns; }, [](const PP& p, const PS&) { return p.n; }, "k_sweep_mark32", num_label);
+    group(BK_REFINE64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_refine64", num_label);
+    group(BK_FULL64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_full64", num_label);
+    group(BK_SCAN32, [](const PP&) { return 0; }, [](const PP& p, const PS&) { return p.n; }, "k_scan32", num_label);
   } else {
-    group(BK_FIT_FP64, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fit_fp64");
-    group(BK_SWEEP_FP64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return p.n * cdiv(p.F, 64); }, "k_sweep_fp64");
-    group(BK_SCAN_FP64, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_scan_fp64");
+    group(BK_FIT_FP64, nb, [&](const PP& p, const PS&) { return cdiv(p.n, 4); }, "k_fit_fp64", num_label);
+    group(BK_SWEEP_FP64, [](const PP& p) { return p.ns; }, [&](const PP& p, const PS&) { return p.n * cdiv(p.F, 64); }, "k_sweep_fp64", num_label);
+    group(BK_SCAN_FP64, [](const PP&) { return 0; }, [&](const PP& p, const PS&) { return cdiv(p.n, 64); }, "k_scan_fp64", num_label);
   }
   // tables: start + pid of every launch in one buffer
   size_t tw = 0;
-  for (auto& L : b->launches) tw += L.start.size() + L.pid.size();
+  for (auto& L : b->launches) tw += L.start.size() + L.pid.size() + L.sid.size();
   std::vector<int32_t> tab;
   tab.reserve(tw);
   std::vector<size_t> toff;
@@ -416,12 +463,14 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     toff.push_back(tab.size());
     tab.insert(tab.end(), L.start.begin(), L.start.end());
     tab.insert(tab.end(), L.pid.begin(), L.pid.end());
+    tab.insert(tab.end(), L.sid.begin(), L.sid.end());
   }
   int32_t* d_tab = (int32_t*)b->tabs.get(tab.size() * 4);
   for (size_t i = 0; i < b->launches.size(); i++) {
     Launch& L = b->launches[i];
     L.d_start = d_tab + toff[i];
     L.d_pid = d_tab + toff[i] + L.start.size();
+    L.d_sid = L.sid.empty() ? nullptr : d_tab + toff[i] + L.start.size() + L.pid.size();
     L.grid = L.start.back();
   }
   BCHK(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
@@ -527,9 +576,11 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
     int grid = grids ? (*grids)[i] : L.grid;
     grid = std::max(1, std::min(grid, total));
     if (L.kind == SIM_LAUNCH) {
-      if (!jit_launch_sim_batch(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
-                                (const char* const*)b->ptrs.p, (int)L.pid.size(), total))
-        throw std::runtime_error("launch of the batched simulation kernel failed");
+      const bool ok = L.d_sid ? jit_launch_sim_batch_fused(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
+                                                           (const char* const*)b->ptrs.p, L.d_sid, (int)L.pid.size(), total)
+                              : jit_launch_sim_batch(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
+                                                     (const char* const*)b->ptrs.p, (int)L.pid.size(), total);
+      if (!ok) throw std::runtime_error("launch of the batched simulation kernel failed");
     } else {
       BatchTab T{L.d_start, L.d_pid, probs, bufs, (int32_t)L.pid.size(), total};
       const bool ok = (L.kind >= BK_ATB) ? launch_batch_mixed(L.kind, L.key, T, grid, ls)
@@ -548,40 +599,83 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
   BCHK(hipGetLastError());
 }
 
-// Whole-chain passes with each candidate grid policy; keeps, per launch, the fastest grid.
-static void autotune(ce_batch* b, int precision, const float* d_in, float* d_out, hipStream_t st) {
+// Whole-chain passes, one stream, every launch timed by events.  For each stage plan (tile: the
+// eigen stages' lanes per candidate, one launch per capacity or one fused launch; the simulation
+// per structure or fused) the chain runs with each grid cap of a persistent launch (at most 8, 16,
+// 32 workgroups per CU, or one per block: the waves each launch keeps in flight); every launch keeps
+// its fastest grid, and every stage the plan whose launches add up to the least time.
+static void autotune(ce_batch* b, int precision, const float* d_gains, const float* d_in, float* d_out, hipStream_t st) {
   int nsm = 256;
   hipDeviceGetAttribute(&nsm, hipDeviceAttributeMultiprocessorCount, b->device);
   const int caps[] = {0, 8 * nsm, 16 * nsm, 32 * nsm};  // 0: one workgroup per block
-  const size_t NL = b->launches.size();
-  std::vector<double> best(NL, 1e300);
-  std::vector<int> pick(NL);
   const int saved = b->profiling;
   b->profiling = 1;
-  for (int c = 0; c < 4; c++) {
-    std::vector<int> grids(NL);
-    for (size_t i = 0; i < NL; i++) {
-      const int total = b->launches[i].start.back();
-      grids[i] = caps[c] ? std::min(total, caps[c]) : total;
+  std::map<std::string, std::pair<double, int>> best;  // launch label -> (time, grid)
+  const int nplans = b->plan_fixed ? 1 : 3;
+  double stage_t[3][3];
+  for (auto& r : stage_t) r[0] = r[1] = r[2] = 1e300;
+  for (int pl = 0; pl < nplans; pl++) {
+    if (!b->plan_fixed) {
+      b->plan[0] = b->plan[1] = pl;
+      b->plan[2] = pl == 2 ? 1 : 0;
+      build_layout(b, precision, st);
+      set_gains(b, d_gains, st);
     }
-    for (int rep = 0; rep < 2; rep++) {  // first pass warms caches and code
-      enqueue(b, precision, d_in, d_out, nullptr, st, &grids);
-      BCHK(hipStreamSynchronize(st));
-    }
-    // marks: start, fill, then one per non-empty launch
-    int m = 2;
-    for (size_t i = 0; i < NL; i++) {
-      if (b->launches[i].start.back() == 0) continue;
-      float t = 0.0f;
-      BCHK(hipEventElapsedTime(&t, b->mark_ev[m - 1], b->mark_ev[m]));
-      m++;
-      if (t < best[i]) {
-        best[i] = t;
-        pick[i] = grids[i];
+    const size_t NL = b->launches.size();
+    std::vector<double> tb(NL, 1e300);
+    std::vector<int> gb(NL, 0);
+    for (int c = 0; c < 4; c++) {
+      std::vector<int> grids(NL);
+      for (size_t i = 0; i < NL; i++) {
+        const int total = b->launches[i].start.back();
+        grids[i] = caps[c] ? std::min(total, caps[c]) : total;
+      }
+      for (int rep = 0; rep < 2; rep++) {  // first pass warms caches and code
+        enqueue(b, precision, d_in, d_out, nullptr, st, &grids);
+        BCHK(hipStreamSynchronize(st));
+      }
+      // marks: start, fill, then one per non-empty launch
+      int m = 2;
+      for (size_t i = 0; i < NL; i++) {
+        if (b->launches[i].start.back() == 0) continue;
+        float t = 0.0f;
+        BCHK(hipEventElapsedTime(&t, b->mark_ev[m - 1], b->mark_ev[m]));
+        m++;
+        if (t < tb[i]) {
+          tb[i] = t;
+          gb[i] = grids[i];
+        }
       }
     }
+    double st_sum[3] = {0.0, 0.0, 0.0};
+    for (size_t i = 0; i < NL; i++) {
+      const Launch& L = b->launches[i];
+      const double t = L.start.back() == 0 ? 0.0 : tb[i];
+      auto it = best.find(L.label);
+      if (it == best.end() || t < it->second.first) best[L.label] = {t, std::max(1, gb[i])};
+      if (L.kind == BK_EIG_LIN) st_sum[0] += t;
+      else if (L.kind == BK_EIG_NOM) st_sum[1] += t;
+      else if (L.kind == SIM_LAUNCH) st_sum[2] += t;
+    }
+    stage_t[0][pl] = st_sum[0];
+    stage_t[1][pl] = st_sum[1];
+    stage_t[2][pl] = st_sum[2];
   }
-  for (size_t i = 0; i < NL; i++) b->launches[i].grid = std::max(1, pick[i]);
+  if (!b->plan_fixed) {
+    for (int k = 0; k < 2; k++) {
+      int bp = 0;
+      for (int pl = 1; pl < 3; pl++)
+        if (stage_t[k][pl] < stage_t[k][bp]) bp = pl;
+      b->plan[k] = bp;
+    }
+    b->plan[2] = stage_t[2][2] < std::min(stage_t[2][0], stage_t[2][1]) ? 1 : 0;
+    build_layout(b, precision, st);
+    set_gains(b, d_gains, st);
+  }
+  for (auto& L : b->launches) {
+    auto it = best.find(L.label);
+    if (it != best.end()) L.grid = it->second.second;
+  }
   b->profiling = saved;
   b->tuned = true;
 }
@@ -603,7 +697,7 @@ static void run(ce_batch* b, int precision, const float* d_gains, const float* d
   upload_blobs(b, st);
   set_gains(b, d_gains, st);
   if (b->autotune && !b->tuned) {
-    autotune(b, precision, d_in, d_out, st);
+    autotune(b, precision, d_gains, d_in, d_out, st);
     drop_graph(b);
   }
   b->last_used_graph = 0;
@@ -769,6 +863,29 @@ extern "C" CE_EXPORT int ce_batch_run(ce_batch* b, int precision, const float* d
     put_err_b(err, errlen, e.what());
     return 1;
   }
+}
+
+extern "C" CE_EXPORT int ce_batch_set_plan(ce_batch* b, int eig_lin, int eig_nom, int sim) {
+  if (!b) return 1;
+  if (eig_lin < 0 || eig_nom < 0 || sim < 0) {
+    b->plan_fixed = false;
+    b->tuned = false;
+    return 0;
+  }
+  if (eig_lin > 2 || eig_nom > 2 || sim > 1) return 1;
+  b->plan[0] = eig_lin;
+  b->plan[1] = eig_nom;
+  b->plan[2] = sim;
+  b->plan_fixed = true;
+  b->layout_dirty = true;
+  b->tuned = false;
+  return 0;
+}
+
+extern "C" CE_EXPORT int ce_batch_plan(ce_batch* b, int* plans) {
+  if (!b || !plans) return 1;
+  for (int k = 0; k < 3; k++) plans[k] = b->plan[k];
+  return 0;
 }
 
 extern "C" CE_EXPORT int ce_batch_info(ce_batch* b, int* launches, int* used_graph, int* tuned) {

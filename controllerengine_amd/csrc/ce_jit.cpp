@@ -615,6 +615,39 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
   return o.str();
 }
 
+// Every structure of a batch in one kernel: the structural body of each structure, and a block
+// loop that finds its problem and runs the body of the problem's structure (sid: structure index
+// of each problem of the launch, in the launch's problem order).
+std::string gen_sim_batch_fused_source(const std::vector<const ProblemDev*>& reps, bool mixed) {
+  std::ostringstream o;
+  const char* name = mixed ? "ce_simb_f32_all" : "ce_simb_f64_all";
+  o << "// kernel: " << name << "\n";
+  o << "// generated: closed-loop simulation for every problem of a batch, " << reps.size() << " structures"
+    << (mixed ? ", float" : ", double") << "\n";
+  o << kSimArgs;
+  for (size_t k = 0; k < reps.size(); k++) {
+    o << "// structure " << k << ": ns=" << reps[k]->ns << " nd=" << reps[k]->nd << " nw=" << reps[k]->nw << "\n";
+    o << "static __device__ inline __attribute__((always_inline)) void sim_s" << k
+      << "(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
+    gen_struct_body(o, *reps[k], mixed);
+    o << "}\n";
+  }
+  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
+    << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
+       "    const char* const* __restrict__ probs, const int* __restrict__ sid, int J, int total) {\n"
+       "  __builtin_amdgcn_s_setprio(3);\n"
+       "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
+       "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
+       "    int lo = 0, hi = J - 1;\n"
+       "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
+       "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
+       "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
+       "    switch (__builtin_amdgcn_readfirstlane(sid[lo])) {\n";
+  for (size_t k = 0; k < reps.size(); k++) o << "      case " << k << ": sim_s" << k << "(args[p], probs[p], lb); break;\n";
+  o << "    }\n  }\n}\n";
+  return o.str();
+}
+
 // ------------------------------------------------------------------------------------------------
 struct JitEntry {
   hipModule_t mod = nullptr;
@@ -757,6 +790,12 @@ bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float
 bool jit_launch_sim_batch(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
                           const char* const* probs, int J, int total) {
   void* a[] = {&start, &pid, &args, &probs, &J, &total};
+  return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, a, nullptr) == hipSuccess;
+}
+
+bool jit_launch_sim_batch_fused(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
+                                const char* const* probs, const int* sid, int J, int total) {
+  void* a[] = {&start, &pid, &args, &probs, &sid, &J, &total};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, a, nullptr) == hipSuccess;
 }
 

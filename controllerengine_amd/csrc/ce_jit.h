@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "ce_common.h"
 
@@ -60,5 +61,10 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed);
 std::string sim_structure_key(const ProblemDev& P, bool mixed);
 bool jit_launch_sim_batch(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
                           const char* const* probs, int J, int total);
+// Every structure of a batch in one kernel (reps: one problem of each structure); sid gives the
+// structure index of each problem of the launch.
+std::string gen_sim_batch_fused_source(const std::vector<const ProblemDev*>& reps, bool mixed);
+bool jit_launch_sim_batch_fused(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
+                                const char* const* probs, const int* sid, int J, int total);
 
 }  // namespace ce

@@ -84,16 +84,21 @@ __global__ __launch_bounds__(64) void k_compact_batch(BatchTab T) {
 // eigensolver.  k_eig_lin (G R >= nt) solves LIN_A and derives modes and flags; k_eig_nom
 // (G R >= 2 ns) solves the nominal loop of the candidates that need it.
 #define CE_ROWS(r) _Pragma("unroll") for (int q_ = 0, r = l; q_ < R; q_++, r += G)
+// LDS of a single-wave workgroup of the eigen kernels: CPW matrices of N x (N + 1), then CPW
+// vectors of 3 N (a fused kernel passes one array sized for its largest tile)
 template <int G, int R>
-__device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+constexpr int eig_lds_doubles() {
+  return (64 / G) * ((G * R) * (G * R + 1) + 3 * G * R);
+}
+
+template <int G, int R>
+__device__ __forceinline__ void k_eig_lin_body(double* lds, const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int N = G * R, S = N + 1, CPW = 64 / G;
-  __shared__ double smat[CPW][N * S];
-  __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
   const int c = bx * CPW + gi;
   if (c >= B.n) return;
-  double* T = smat[gi];
-  double* V = svec[gi];
+  double* T = lds + gi * (N * S);
+  double* V = lds + CPW * (N * S) + gi * (3 * N);
   double* WR = V + N;
   double* WI = V + 2 * N;
   // after the solve: |lambda| in V (the reduction's scratch), the continuous-time mode (re, im) of
@@ -200,14 +205,16 @@ __device__ __forceinline__ void k_eig_lin_body(const ProblemDev* __restrict__ P,
 
 template <int G, int R>
 __global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_eig_lin_body<G, R>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+  __shared__ double lds[eig_lds_doubles<G, R>()];
+  k_eig_lin_body<G, R>(lds, P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 template <int G, int R>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_lin_batch(BatchTab T) {
+  __shared__ double lds[eig_lds_doubles<G, R>()];
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
-    k_eig_lin_body<G, R>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+    k_eig_lin_body<G, R>(lds, T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
   }
 }
 
@@ -216,10 +223,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // candidates of the nominal list (k_compact): linearly stable, not rejected, not decided by the
 // LIN_A solve
 template <int G, int R>
-__device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
+__device__ __forceinline__ void k_eig_nom_body(double* lds, const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   constexpr int N = G * R, S = N + 1, CPW = 64 / G;
-  __shared__ double smat[CPW][N * S];
-  __shared__ double svec[CPW][3 * N];
   const int gi = threadIdx.x / G, l = threadIdx.x % G;
   // candidates from the nominal list (after k_compact), so that the groups of a wave all solve
   const int idx = bx * CPW + gi;
@@ -227,8 +232,8 @@ __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P,
   const int c = B.nlist[idx];
   const int flags = B.flags[c];
   if (flags & F_NOMINAL_SKIPPED) return;
-  double* T = smat[gi];
-  double* V = svec[gi];
+  double* T = lds + gi * (N * S);
+  double* V = lds + CPW * (N * S) + gi * (3 * N);
   double* WR = V + N;
   double* WI = V + 2 * N;
   const int ns = P->ns, m = 2 * ns;
@@ -256,14 +261,44 @@ __device__ __forceinline__ void k_eig_nom_body(const ProblemDev* __restrict__ P,
 
 template <int G, int R>
 __global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P, StageBufs B) {
-  k_eig_nom_body<G, R>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
+  __shared__ double lds[eig_lds_doubles<G, R>()];
+  k_eig_nom_body<G, R>(lds, P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 template <int G, int R>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_nom_batch(BatchTab T) {
+  __shared__ double lds[eig_lds_doubles<G, R>()];
   for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
     const BatchSlot q = batch_slot(T, b);
-    k_eig_nom_body<G, R>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+    k_eig_nom_body<G, R>(lds, T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
+  }
+}
+
+// One launch for every problem of the stage, whatever its order: each block runs the default tile
+// of its problem's matrix capacity (orders up to 4, 8, 16) on one LDS array sized for the largest.
+constexpr int EIG_FUSED_LDS = eig_lds_doubles<4, 1>() > eig_lds_doubles<EIG8_G, 8 / EIG8_G>()
+                                  ? (eig_lds_doubles<4, 1>() > eig_lds_doubles<EIG16_G, 16 / EIG16_G>()
+                                         ? eig_lds_doubles<4, 1>() : eig_lds_doubles<EIG16_G, 16 / EIG16_G>())
+                                  : (eig_lds_doubles<EIG8_G, 8 / EIG8_G>() > eig_lds_doubles<EIG16_G, 16 / EIG16_G>()
+                                         ? eig_lds_doubles<EIG8_G, 8 / EIG8_G>() : eig_lds_doubles<EIG16_G, 16 / EIG16_G>());
+
+template <bool NOM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_eig_batch_fused(BatchTab T) {
+  __shared__ double lds[EIG_FUSED_LDS];
+  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
+    const BatchSlot q = batch_slot(T, b);
+    const ProblemDev* P = T.probs[q.p];
+    const int key = eig_key(NOM ? 2 * P->ns : P->nt);
+    if (key == 4) {
+      if (NOM) k_eig_nom_body<4, 1>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+      else k_eig_lin_body<4, 1>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+    } else if (key == 8) {
+      if (NOM) k_eig_nom_body<EIG8_G, 8 / EIG8_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+      else k_eig_lin_body<EIG8_G, 8 / EIG8_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+    } else {
+      if (NOM) k_eig_nom_body<EIG16_G, 16 / EIG16_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+      else k_eig_lin_body<EIG16_G, 16 / EIG16_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
+    }
   }
 }
 #undef CE_ROWS
@@ -679,18 +714,27 @@ bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream
   }                                                                         \
   return false;
   switch (kind) {
+    // eigen keys: 0 = the fused launch over every order; else capacity * 100 + lanes per candidate
     case BK_EIG_LIN:
-      if (key == 4) hipLaunchKernelGGL((k_eig_lin_batch<4, 1>), g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL((k_eig_lin_batch<EIG8_G, 8 / EIG8_G>), g, b, 0, st, T);
-      else if (key == 16) hipLaunchKernelGGL((k_eig_lin_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
-      else return false;
-      return true;
+      switch (key) {
+        case 0: hipLaunchKernelGGL(k_eig_batch_fused<false>, g, b, 0, st, T); return true;
+        case 404: hipLaunchKernelGGL((k_eig_lin_batch<4, 1>), g, b, 0, st, T); return true;
+        case 804: hipLaunchKernelGGL((k_eig_lin_batch<4, 2>), g, b, 0, st, T); return true;
+        case 808: hipLaunchKernelGGL((k_eig_lin_batch<8, 1>), g, b, 0, st, T); return true;
+        case 1608: hipLaunchKernelGGL((k_eig_lin_batch<8, 2>), g, b, 0, st, T); return true;
+        case 1616: hipLaunchKernelGGL((k_eig_lin_batch<16, 1>), g, b, 0, st, T); return true;
+      }
+      return false;
     case BK_EIG_NOM:
-      if (key == 4) hipLaunchKernelGGL((k_eig_nom_batch<4, 1>), g, b, 0, st, T);
-      else if (key == 8) hipLaunchKernelGGL((k_eig_nom_batch<EIG8_G, 8 / EIG8_G>), g, b, 0, st, T);
-      else if (key == 16) hipLaunchKernelGGL((k_eig_nom_batch<EIG16_G, 16 / EIG16_G>), g, b, 0, st, T);
-      else return false;
-      return true;
+      switch (key) {
+        case 0: hipLaunchKernelGGL(k_eig_batch_fused<true>, g, b, 0, st, T); return true;
+        case 404: hipLaunchKernelGGL((k_eig_nom_batch<4, 1>), g, b, 0, st, T); return true;
+        case 804: hipLaunchKernelGGL((k_eig_nom_batch<4, 2>), g, b, 0, st, T); return true;
+        case 808: hipLaunchKernelGGL((k_eig_nom_batch<8, 1>), g, b, 0, st, T); return true;
+        case 1608: hipLaunchKernelGGL((k_eig_nom_batch<8, 2>), g, b, 0, st, T); return true;
+        case 1616: hipLaunchKernelGGL((k_eig_nom_batch<16, 1>), g, b, 0, st, T); return true;
+      }
+      return false;
     case BK_COMPACT: hipLaunchKernelGGL(k_compact_batch, g, b, 0, st, T); return true;
     case BK_FIT_FP64:
       switch (key) {

@@ -75,7 +75,7 @@ struct StageBufs {
 // eigensolve 0.49 vs 0.38 ms: twice the LDS per wave, half the waves in flight).
 constexpr int EIG16_G = 16;
 constexpr int EIG8_G = 4;
-inline int eig_key(int order) { return order <= 4 ? 4 : (order <= 8 ? 8 : 16); }
+__host__ __device__ inline int eig_key(int order) { return order <= 4 ? 4 : (order <= 8 ? 8 : 16); }
 inline int eig_lanes(int key) { return key == 16 ? EIG16_G : (key == 8 ? EIG8_G : key); }
 
 // Batched launches: one launch of a stage kernel covers a group of problems (same kernel

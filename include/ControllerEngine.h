@@ -202,11 +202,20 @@ CE_EXPORT int ce_batch_update(ce_batch* b, int index, const ce_config* cfg, cons
                               int errlen);
 CE_EXPORT int ce_batch_offsets(ce_batch* b, int index, long long* gains_off, long long* info_off);
 /* use_graph: capture the stage chain into a hipGraph and replay it (default on); autotune: choose
- * each launch's grid size by timing whole-chain passes on the next run (results do not depend on
- * it); profiling: record an event after every launch (disables the graph). */
+ * each stage's plan (ce_batch_set_plan) and each launch's grid size (the waves a persistent launch
+ * keeps in flight) by timing whole-chain passes on the next run (results do not depend on it);
+ * profiling: record an event after every launch (disables the graph). */
 CE_EXPORT int ce_batch_set_options(ce_batch* b, int use_graph, int autotune, int profiling);
 CE_EXPORT int ce_batch_run(ce_batch* b, int precision, const float* d_gains, const float* d_info_in, float* d_info_out,
                            double* d_info64_out, void* stream, char* err, int errlen);
+/* Stage plans of the batched chain (no counterpart in the reference).  eig_lin, eig_nom (the LIN_A and
+ * nominal eigensolves): 0 one launch per matrix capacity (orders <= 4, 8, 16) with the default lanes
+ * per candidate (4, 4, 16), 1 the same with the alternative lanes (8 at orders 5-16), 2 one fused launch
+ * over every order.  sim: 0 one simulation launch per instruction structure, 1 one fused launch.  Rows
+ * do not depend on the plan.  Fixed plans are kept by the autotuner; a negative value hands the choice
+ * back to it.  ce_batch_plan reads the plans in use (eig_lin, eig_nom, sim). */
+CE_EXPORT int ce_batch_set_plan(ce_batch* b, int eig_lin, int eig_nom, int sim);
+CE_EXPORT int ce_batch_plan(ce_batch* b, int* plans);
 /* Number of grouped launches, whether the last run replayed a graph, whether the grids are tuned. */
 CE_EXPORT int ce_batch_info(ce_batch* b, int* launches, int* used_graph, int* tuned);
 /* As ce_engine_kernel_times, for the last run with profiling on. */

@@ -69,6 +69,9 @@ def test_batch_autotune_and_update(engine, batch):
     print("batch autotune", flush=True)
     outs = b.evaluate(PRECISION_MIXED)
     assert b.info()["tuned"]
+    plan = b.plan()
+    print("tuned plan", plan, flush=True)
+    assert plan["eig_lin"] in (0, 1, 2) and plan["eig_nom"] in (0, 1, 2) and plan["sim"] in (0, 1)
     for p, (o32, o64) in zip(ps, outs):
         s32, s64 = single(engine, p, PRECISION_MIXED)
         assert np.array_equal(o64.view(np.uint64), s64.view(np.uint64)), p.name
@@ -84,6 +87,30 @@ def test_batch_autotune_and_update(engine, batch):
     assert np.array_equal(outs[1][1].view(np.uint64), s64.view(np.uint64))
     b.update(1, ps[1])
     b.set_options(use_graph=True, autotune=False)
+
+
+@pytest.mark.parametrize("plan", [(1, 1, 0), (2, 2, 1), (0, 2, 0), (2, 1, 1)])
+@pytest.mark.parametrize("prec", [PRECISION_FP64, PRECISION_MIXED])
+def test_batch_plans_equal_single(engine, prec, plan):
+    """Every stage plan gives the single chain's rows bit for bit: the eigen stages with 8 lanes per
+    candidate at orders 5-16 (plan 1) or as one fused launch over orders 4, 8 and 16 (plan 2), and
+    the simulation as one fused launch over every structure (sim plan 1)."""
+    from controllerengine_amd.batch import Batch
+    b = Batch(0)
+    try:
+        ps = problems()
+        for p in ps:
+            b.add(p)
+        b.set_plan(*plan)
+        b.set_options(use_graph=True, autotune=False)
+        print("batch plan", plan, "prec", prec, flush=True)
+        outs = b.evaluate(prec)
+        assert b.plan() == dict(eig_lin=plan[0], eig_nom=plan[1], sim=plan[2])
+        for p, (o32, o64) in zip(ps, outs):
+            s32, s64 = single(engine, p, prec)
+            assert np.array_equal(o64.view(np.uint64), s64.view(np.uint64)), (p.name, plan)
+    finally:
+        b.close()
 
 
 def order_jobs():
