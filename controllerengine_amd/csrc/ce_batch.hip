@@ -79,6 +79,7 @@ struct Launch {
   int32_t* d_pid = nullptr;
   int32_t* d_sid = nullptr;
   void* fn = nullptr;  // simulation kernel
+  void* fl[2] = {nullptr, nullptr};  // its 2- and 4-lane forms (float)
   int grid = 0;        // chosen grid
   std::string label;
 };
@@ -144,6 +145,7 @@ struct ce_batch {
   // The autotuner picks them unless a caller fixed them (ce_batch_set_plan).
   int plan[3] = {0, 0, 0};
   bool plan_fixed = false;
+  int sim_lanes = 1;  // lanes per candidate of the simulation launches (float form)
   // per-launch timing
   int profiling = 0;
   std::vector<hipEvent_t> mark_ev;
@@ -389,7 +391,26 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
   group(BK_EIG_NOM, [&](const PP& p) { return eig_tile(eig_key(2 * p.ns), b->plan[1]); },
         [&](const PP& p, const PS&) { return eig_blocks(2 * p.ns, b->plan[1], p.n); }, "k_eig_nom", eig_label);
   {  // simulation: one group per structure, its problems longest horizon first (a wave's work is
-     // proportional to the horizon: the long ones start first instead of forming the tail)
+     // proportional to the horizon: the long ones start first instead of forming the tail).  The
+     // float form splits a candidate's step over 2 or 4 lanes when the batch's candidates would
+     // leave SIMDs idle at one lane each (serving batches of small jobs).
+    int nsimd = 1024;
+    {
+      int cus = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && cus > 0)
+        nsimd = 4 * cus;
+    }
+    int lanes = 1;
+    if (mixed)
+      for (int G : {4, 2}) {
+        long long waves = 0;
+        for (int j = 0; j < J; j++) waves += cdiv(b->P[j].n, 64 / G);
+        if (waves <= nsimd) {
+          lanes = G;
+          break;
+        }
+      }
+    b->sim_lanes = lanes;
     std::vector<int> order(J);
     for (int j = 0; j < J; j++) order[j] = j;
     auto horizon = [&](int j) { return reinterpret_cast<const ProblemDev*>(b->P[j].blob.data())->T; };
@@ -411,10 +432,11 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
         }
         L.pid.push_back(j);
         L.sid.push_back(it->second);
-        L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64));
+        L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64 / lanes));
       }
       std::string e;
-      if (!jit_get(shared_jit(b->device), gen_sim_batch_fused_source(reps, mixed), &L.fn, e)) throw std::runtime_error(e);
+      if (!jit_get(shared_jit(b->device), gen_sim_batch_fused_source(reps, mixed), &L.fn, e, false, L.fl))
+        throw std::runtime_error(e);
       L.label = std::string(mixed ? "ce_simb_f32" : "ce_simb_f64") + "/all";
       b->launches.push_back(L);
     }
@@ -428,10 +450,11 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
         L.skey = k;
         L.start.push_back(0);
         std::string e;
-        if (!jit_get(shared_jit(b->device), gen_sim_batch_source(H, mixed), &L.fn, e)) throw std::runtime_error(e);
+        if (!jit_get(shared_jit(b->device), gen_sim_batch_source(H, mixed), &L.fn, e, false, L.fl))
+          throw std::runtime_error(e);
       }
       L.pid.push_back(j);
-      L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64));
+      L.start.push_back(L.start.back() + cdiv(b->P[j].n, 64 / lanes));
     }
     int i = 0;
     for (auto& kv : g) {
@@ -576,9 +599,11 @@ static void enqueue(ce_batch* b, int precision, const float* d_in, float* d_out,
     int grid = grids ? (*grids)[i] : L.grid;
     grid = std::max(1, std::min(grid, total));
     if (L.kind == SIM_LAUNCH) {
-      const bool ok = L.d_sid ? jit_launch_sim_batch_fused(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
+      void* f = b->sim_lanes == 4 ? L.fl[1] : (b->sim_lanes == 2 ? L.fl[0] : L.fn);
+      if (!f) throw std::runtime_error("no " + std::to_string(b->sim_lanes) + "-lane simulation kernel");
+      const bool ok = L.d_sid ? jit_launch_sim_batch_fused(f, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
                                                            (const char* const*)b->ptrs.p, L.d_sid, (int)L.pid.size(), total)
-                              : jit_launch_sim_batch(L.fn, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
+                              : jit_launch_sim_batch(f, grid, ls, L.d_start, L.d_pid, (const SimArgs*)b->sargs.p,
                                                      (const char* const*)b->ptrs.p, (int)L.pid.size(), total);
       if (!ok) throw std::runtime_error("launch of the batched simulation kernel failed");
     } else {

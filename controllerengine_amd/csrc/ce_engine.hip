@@ -88,6 +88,7 @@ struct ce_engine {
   void* sim_fn[2][2] = {};
   bool sim_fn_known[2][2] = {};
   bool sim_fn_struct[2][2] = {};  // the structural form (constants from the descriptor)
+  void* sim_fn_lanes[2][2][2] = {};  // the literal float module's 2- and 4-lane kernels
   // structures (sim_structure_key, gated or not) whose literal kernel this engine has compiled: a
   // second problem of the same structure means the constants change between calls (serial rounds,
   // a search over plants), and it gets the structural kernel, compiled once per structure
@@ -285,10 +286,12 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
         const std::string skey = sim_structure_key(H, mixed) + (g ? "|gated" : "");
         const bool structural = !jit_cached(E->jit, lit) && E->literal_structs.count(skey) != 0;
         bool ok;
+        void** lanes = E->sim_fn_lanes[mi][gi];
+        lanes[0] = lanes[1] = nullptr;
         if (structural) {
           ok = jit_get(E->jit, gen_sim_struct_source(H, mixed, g), &f, jerr, false);
         } else {
-          ok = jit_get(E->jit, lit, &f, jerr, true);
+          ok = jit_get(E->jit, lit, &f, jerr, true, lanes);
           if (ok) E->literal_structs.insert(skey);
         }
         E->sim_fn[mi][gi] = ok ? f : nullptr;
@@ -344,19 +347,28 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   if (fn) {
     const bool st_form = E->sim_fn_struct[mixed ? 1 : 0][gated ? 1 : 0];
     bool ok;
-    // lanes per candidate the float literal kernel may use: a grid of at most two waves per SIMD
-    // (the kernel picks the lanes from the live count, keeping its working waves one per SIMD)
+    // lanes per candidate the float literal kernels may use: grids of at most two waves per SIMD.
+    // The 4-, 2- and 1-lane kernels are launched in turn, as the grid allows; each runs only if the
+    // live count selects its lanes (the most whose working waves stay one per SIMD).
+    void* const* lanes = E->sim_fn_lanes[mixed ? 1 : 0][gated ? 1 : 0];
     int gq = 1;
-    if (mixed && !st_form) gq = (4 * sim_grid <= 2 * E->n_simd) ? 4 : ((2 * sim_grid <= 2 * E->n_simd) ? 2 : 1);
-    if (st_form)
+    if (mixed && !st_form && lanes[0] && lanes[1])
+      gq = (4 * sim_grid <= 2 * E->n_simd) ? 4 : ((2 * sim_grid <= 2 * E->n_simd) ? 2 : 1);
+    if (st_form) {
       ok = gated ? jit_launch_sim_struct_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
                                                n, (int*)E->gate.p, E->gate_gen, P)
                  : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P);
-    else
-      ok = gated ? jit_launch_sim_gated(fn, gq * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
-                                        (int*)E->gate.p, E->gate_gen, gq, E->n_simd)
-                 : jit_launch_sim(fn, gq * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, gq,
-                                  E->n_simd);
+    } else {
+      ok = true;
+      for (int G : {4, 2, 1}) {
+        if (G > gq) continue;
+        void* f = G == 4 ? lanes[1] : (G == 2 ? lanes[0] : fn);
+        ok = ok && (gated ? jit_launch_sim_gated(f, G * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
+                                                 n, (int*)E->gate.p, E->gate_gen, gq, E->n_simd)
+                          : jit_launch_sim(f, G * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
+                                           gq, E->n_simd));
+      }
+    }
     if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");
     E->last_sim_name = st_form ? (mixed ? (gated ? "ce_sims_f32_g" : "ce_sims_f32") : (gated ? "ce_sims_f64_g" : "ce_sims_f64"))
                                : (mixed ? (gated ? "ce_sim_f32_g" : "ce_sim_f32") : (gated ? "ce_sim_f64_g" : "ce_sim_f64"));

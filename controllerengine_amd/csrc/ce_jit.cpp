@@ -347,41 +347,58 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
 #undef CE_OFF
 }
 
-// Lane-group form of the float literal kernel: G lanes (2 or 4) per candidate, for blocks whose
-// live candidates leave SIMDs idle at one lane each.  Every lane of a group runs the instruction
+// Lane-group form of the float kernels: G lanes (2 or 4) per candidate, for launches whose
+// candidates leave SIMDs idle at one lane each.  Every lane of a group runs the instruction
 // chains, K x and C x over the whole state, as the one-lane form does; lane j updates the rows
 // G s + j of the observer and plant state (the same terms in the same order, then the compensated
 // add), and each group's new rows are broadcast to its lanes with DPP quad permutes.  Every row
 // therefore sees the one-lane form's operations: the trajectory is bit-identical.  A lane's row
-// coefficients are selected once, before the time loop.
-static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
+// coefficients are selected once, before the time loop: literals, or (structural form) loads from
+// the problem descriptor Pp, whose zero pattern is part of the structure.
+static void gen_body_grp(Gen& g, const ProblemDev& P, int G, bool structural) {
   const int ns = P.ns, nd = P.nd, nw = P.nw;
   const int lg = G == 4 ? 2 : 1;
   std::ostringstream& o = g.o;
   std::ostringstream pre;
-  int nsel = 0;
-  auto all_zero = [](const std::vector<double>& v) {
-    for (double x : v)
-      if ((float)x != 0.0f) return false;
-    return true;
+  int nsel = 0, nk = 0;
+#define CE_OFF(f) offsetof(ProblemDev, f)
+  auto ldp = [](const char* ty, size_t off) {
+    return std::string("(*(const __attribute__((address_space(4))) ") + ty + "*)((const __attribute__((address_space(4))) char*)Pp + " +
+           std::to_string(off) + "))";
   };
-  // this lane's value of a per-row coefficient (v[j]: row G s + j; absent rows 0)
-  auto sel = [&](const std::vector<double>& v) -> std::string {
-    bool same = true;
-    for (int j = 1; j < G; j++)
-      if ((float)v[j] != (float)v[0]) same = false;
-    if (same) return g.L(v[0]);
-    const std::string nm = "pc" + std::to_string(nsel++);
-    pre << "  const float " << nm << " = ";
-    for (int j = G - 1; j >= 1; j--) pre << "h == " << j << " ? " << g.L(v[j]) << " : ";
-    pre << g.L(v[0]) << ";\n";
+  // a wave-uniform constant of the step: a literal, or loaded once before the loop
+  auto cst = [&](double v, size_t off) -> std::string {
+    if (!structural) return g.L(v);
+    const std::string nm = "kc" + std::to_string(nk++);
+    pre << "  const float " << nm << " = " << ldp("float", off) << ";\n";
     return nm;
   };
-  auto rowv = [&](int s, int n, auto f) {
-    std::vector<double> v(G, 0.0);
+  auto ival = [&](int v, size_t off) { return structural ? ldp("int", off) : std::to_string(v); };
+  // rows G s + j of a per-row coefficient (fv: value of row r, fo: its float offset)
+  auto zero_rows = [&](int s, int n, auto fv) {
     for (int j = 0; j < G; j++)
-      if (G * s + j < n) v[j] = f(G * s + j);
-    return v;
+      if (G * s + j < n && (float)fv(G * s + j) != 0.0f) return false;
+    return true;
+  };
+  auto sel = [&](int s, int n, auto fv, auto fo) -> std::string {
+    if (!structural) {
+      bool same = true;
+      for (int j = 1; j < G; j++) {
+        const float a = G * s + j < n ? (float)fv(G * s + j) : 0.0f, b = (float)fv(G * s);
+        if (a != b) same = false;
+      }
+      if (same) return g.L(fv(G * s));
+    }
+    auto val = [&](int j) -> std::string {
+      const int r = G * s + j;
+      if (r >= n) return "0.0f";
+      return structural ? ldp("float", fo(r)) : g.L(fv(r));
+    };
+    const std::string nm = "pc" + std::to_string(nsel++);
+    pre << "  const float " << nm << " = ";
+    for (int j = G - 1; j >= 1; j--) pre << "h == " << j << " ? " << val(j) << " : ";
+    pre << val(0) << ";\n";
+    return nm;
   };
   // DPP quad permute that gives every lane of a group the value of the group's lane j
   auto bc = [&](int j) {
@@ -390,13 +407,17 @@ static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
     return "ce_bc<" + std::to_string(p[0] | p[1] << 2 | p[2] << 4 | p[3] << 6) + ">";
   };
   const int S = (ns + G - 1) / G, SP = (nd + G - 1) / G;
+  const std::string sT = ival(P.T, CE_OFF(T)), sF0 = ival(P.fit_start_id, CE_OFF(fit_start_id));
+  const std::string sFC = ival(P.fit_count, CE_OFF(fit_count)), sS0 = ival(P.steady_start_id, CE_OFF(steady_start_id));
+  const std::string sRS = ival(P.ref_step_id, CE_OFF(ref_step_id)), sCW = ival(P.cuda_w, CE_OFF(cuda_w));
+  const std::string sIW = ival(P.info_w, CE_OFF(info_w)), sFR = ival(P.info_freqresp, CE_OFF(info_freqresp));
   o << "  const int h = threadIdx.x & " << G - 1 << ";\n";
   o << "  const int idx = bx * " << 64 / G << " + (threadIdx.x >> " << lg << ");\n  if (idx >= *A.count) return;\n"
        "  const int c = A.list[idx];\n";
-  o << "  const int n = A.n;\n  const int TT = " << P.T << ", f0 = " << P.fit_start_id << ", f1 = "
-    << P.fit_start_id + P.fit_count << ", s0 = " << P.steady_start_id << ", rs = " << P.ref_step_id << ";\n";
+  o << "  const int n = A.n;\n  const int TT = " << sT << ", f0 = " << sF0 << ", f1 = " << sF0 << " + " << sFC
+    << ", s0 = " << sS0 << ", rs = " << sRS << ";\n";
   o << "  const bool unstable = (A.flags[c] & 1) != 0;\n";
-  o << "  const float* g = A.gains + (long)c * " << P.cuda_w << ";\n";
+  o << "  const float* g = A.gains + (long)c * " << sCW << ";\n";
   for (int i = 0; i < ns; i++) o << "  const float K" << i << " = g[" << i << "];\n";
   for (int s = 0; s < S; s++)
     o << "  const float Lo" << s << " = (" << G * s << " + h < " << ns << ") ? g[" << ns + G * s << " + h] : 0.0f;\n";
@@ -407,15 +428,17 @@ static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
   for (int s = 0; s < SP; s++) o << "  float op" << s << " = 0.0f, opl" << s << " = 0.0f;\n";
   o << "  float ysum = 0.0f, ymax = -__builtin_inff(), ymin = __builtin_inff();\n";
   o << "  float* yb = (float*)A.ybuf + idx;\n  float* dy = A.dy;\n";
-  o << "  const float amp = " << g.L(P.ref_amp) << ";\n";
   const std::string mark = "  /*lane rows*/\n";
   o << mark;
+  o << "  const float amp = " << cst(P.ref_amp, CE_OFF(fref_amp)) << ";\n";
   o << "  auto step = [&](const int t, auto ST, auto SS, auto RZ, auto DY) __attribute__((always_inline)) {\n";
   std::vector<std::string> sxp, sxh;
   for (int i = 0; i < nd; i++) sxp.push_back("xp" + std::to_string(i));
   for (int i = 0; i < ns; i++) sxh.push_back("xh" + std::to_string(i));
-  auto litc = [&](int, int, double v) { return g.L(v); };
-  const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "", litc);
+  auto chain_cst = [&](size_t off_a, size_t off_b) {
+    return [&, off_a, off_b](int r, int which, double v) { return cst(v, (which ? off_b : off_a) + (size_t)r * sizeof(float)); };
+  };
+  const std::string y = g.chain(P.dc_ops, P.dc_n, sxp, "", chain_cst(CE_OFF(fdc_aval), CE_OFF(fdc_bval)));
   o << "    const float y = " << y << ";\n";
   o << "    if (DY) dy[(long)c * TT + t] = y;\n";
   o << "    if (ST) yb[(long)(t - f0) * n] = y;\n";
@@ -426,24 +449,29 @@ static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
   for (int k = 0; k < ns; k++) o << "    kx = __builtin_fmaf(K" << k << ", xhl" << k << ", kx);\n";
   o << "    const float rr = RZ ? 0.0f : amp;\n";
   o << "    const float u_raw = __builtin_fmaf(N, rr, -kx);\n";
-  const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", litc);
+  const std::string u = g.chain(P.u_ops, P.u_n, sxh, "u_raw", chain_cst(CE_OFF(fu_aval), CE_OFF(fu_bval)));
   o << "    const float uu = " << u << ";\n";
   o << "    float cx = 0.0f;\n";
+  std::vector<std::string> csc(ns);
   for (int k = 0; k < ns; k++)
-    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sc[k]) << ", xh" << k << ", cx);\n";
+    if ((float)P.sc[k] != 0.0f) csc[k] = cst(P.sc[k], CE_OFF(fsc) + k * 4);
   for (int k = 0; k < ns; k++)
-    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sc[k]) << ", xhl" << k << ", cx);\n";
-  if ((float)P.sd != 0.0f) o << "    cx = __builtin_fmaf(" << g.L(P.sd) << ", uu, cx);\n";
+    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << csc[k] << ", xh" << k << ", cx);\n";
+  for (int k = 0; k < ns; k++)
+    if ((float)P.sc[k] != 0.0f) o << "    cx = __builtin_fmaf(" << csc[k] << ", xhl" << k << ", cx);\n";
+  if ((float)P.sd != 0.0f) o << "    cx = __builtin_fmaf(" << cst(P.sd, CE_OFF(fsd)) << ", uu, cx);\n";
   o << "    const float yerr = y - cx;\n";
   for (int s = 0; s < S; s++) {
-    const auto vb = rowv(s, ns, [&](int r) { return P.sb[r]; });
-    if (all_zero(vb)) o << "    float dh" << s << " = Lo" << s << " * yerr;\n";
-    else o << "    float dh" << s << " = __builtin_fmaf(Lo" << s << ", yerr, " << sel(vb) << " * uu);\n";
+    auto fsb = [&](int r) { return P.sb[r]; };
+    auto osb = [&](int r) { return CE_OFF(fsb) + (size_t)r * 4; };
+    if (zero_rows(s, ns, fsb)) o << "    float dh" << s << " = Lo" << s << " * yerr;\n";
+    else o << "    float dh" << s << " = __builtin_fmaf(Lo" << s << ", yerr, " << sel(s, ns, fsb, osb) << " * uu);\n";
     o << "    float dl" << s << " = 0.0f;\n";
     for (int k = 0; k < ns; k++) {
-      const auto va = rowv(s, ns, [&](int r) { return P.sai[r * ns + k]; });
-      if (all_zero(va)) continue;
-      const std::string cf = sel(va);
+      auto fa = [&](int r) { return P.sai[r * ns + k]; };
+      auto oa = [&](int r) { return CE_OFF(fsai) + (size_t)(r * ns + k) * 4; };
+      if (zero_rows(s, ns, fa)) continue;
+      const std::string cf = sel(s, ns, fa, oa);
       o << "    dh" << s << " = __builtin_fmaf(" << cf << ", xh" << k << ", dh" << s << ");\n";
       o << "    dl" << s << " = __builtin_fmaf(" << cf << ", xhl" << k << ", dl" << s << ");\n";
     }
@@ -457,16 +485,25 @@ static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
   std::vector<std::string> snl = sxp;
   snl.push_back("y");
   std::vector<std::string> W;
-  for (int q = 0; q < nw; q++) W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, "", litc));
+  for (int q = 0; q < nw; q++) {
+    const size_t ofa = CE_OFF(fnl_aval) + (size_t)q * CE_MAX_NL_INSTRUCTIONS * sizeof(float);
+    const size_t ofb = CE_OFF(fnl_bval) + (size_t)q * CE_MAX_NL_INSTRUCTIONS * sizeof(float);
+    W.push_back(g.chain(P.nl_ops[q], P.nl_n[q], snl, "", chain_cst(ofa, ofb)));
+  }
   for (int s = 0; s < SP; s++) {
     o << "    float ph" << s << " = 0.0f, pl" << s << " = 0.0f;\n";
-    auto term = [&](const char* acc, const std::vector<double>& v, const std::string& x) {
-      if (!all_zero(v)) o << "    " << acc << s << " = __builtin_fmaf(" << sel(v) << ", " << x << ", " << acc << s << ");\n";
+    auto term = [&](const char* acc, auto fv, auto fo, const std::string& x) {
+      if (!zero_rows(s, nd, fv)) o << "    " << acc << s << " = __builtin_fmaf(" << sel(s, nd, fv, fo) << ", " << x << ", " << acc << s << ");\n";
     };
-    term("ph", rowv(s, nd, [&](int r) { return P.dbu[r]; }), "uu");
-    for (int q = 0; q < nw; q++) term("ph", rowv(s, nd, [&](int r) { return P.dbw[r * nw + q]; }), W[q]);
-    for (int k = 0; k < nd; k++) term("ph", rowv(s, nd, [&](int r) { return P.dai[r * nd + k]; }), "xp" + std::to_string(k));
-    for (int k = 0; k < nd; k++) term("pl", rowv(s, nd, [&](int r) { return P.dai[r * nd + k]; }), "xpl" + std::to_string(k));
+    term("ph", [&](int r) { return P.dbu[r]; }, [&](int r) { return CE_OFF(fdbu) + (size_t)r * 4; }, "uu");
+    for (int q = 0; q < nw; q++)
+      term("ph", [&](int r) { return P.dbw[r * nw + q]; }, [&](int r) { return CE_OFF(fdbw) + (size_t)(r * nw + q) * 4; }, W[q]);
+    for (int k = 0; k < nd; k++)
+      term("ph", [&](int r) { return P.dai[r * nd + k]; }, [&](int r) { return CE_OFF(fdai) + (size_t)(r * nd + k) * 4; },
+           "xp" + std::to_string(k));
+    for (int k = 0; k < nd; k++)
+      term("pl", [&](int r) { return P.dai[r * nd + k]; }, [&](int r) { return CE_OFF(fdai) + (size_t)(r * nd + k) * 4; },
+           "xpl" + std::to_string(k));
   }
   for (int s = 0; s < SP; s++) o << "    ce_hladd(op" << s << ", opl" << s << ", ph" << s << ", pl" << s << ");\n";
   for (int s = 0; s < SP; s++)
@@ -486,8 +523,9 @@ static void gen_body_grp(Gen& g, const ProblemDev& P, int G) {
   o << "  if (unstable || h != 0) return;\n";
   o << "  const float yss = ysum / (float)(TT - s0);\n";
   o << "  const float pk = ce_maxf(0.0f, ((yss < 0.0f) ? ymin : ymax) * yss);\n";
-  o << "  double* fr = A.info + (long)c * " << P.info_w << " + " << P.info_freqresp << ";\n";
+  o << "  double* fr = A.info + (long)c * " << sIW << " + " << sFR << ";\n";
   o << "  fr[2] = (double)(pk / __builtin_fmaxf(yss * yss, " << litf((float)EPS6) << ") - 1.0f);\n  fr[3] = (double)yss;\n";
+#undef CE_OFF
 }
 
 static const char* kSimArgs =
@@ -558,35 +596,64 @@ static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool 
     for (int G : {2, 4}) {
       Gen gg;
       gg.f32 = true;
-      gen_body_grp(gg, P, G);
+      gen_body_grp(gg, P, G, false);
       o << "static __device__ inline __attribute__((always_inline)) void sim_g" << G
         << "(const SimArgs& A, int bx) {\n" << gg.o.str() << "}\n";
     }
   }
-  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
-    << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
-       "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
-       "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
-    << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap") << ") {\n"
-       "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
-  if (gated)
-    o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
-         "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
-         "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
-  o << "  __builtin_amdgcn_s_setprio(3);\n";
-  if (grp)
-    // lanes per candidate: the most the launch allows (gq) with the live candidates' waves still
-    // at most cap, one per SIMD
-    o << "  const int live = __builtin_amdgcn_readfirstlane(*count);\n"
-         "  if (gq >= 4 && live <= 16 * cap) { sim_g4(A, blockIdx.x); return; }\n"
-         "  if (gq >= 2 && live <= 32 * cap) { sim_g2(A, blockIdx.x); return; }\n";
-  o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
+  // one kernel per lane count (each with its own register allocation): NAME (one lane), NAME_l2 and
+  // NAME_l4.  The host launches the forms the grid allows (gq); each computes the lanes the live
+  // count selects (the most, up to gq, whose waves stay within cap, one per SIMD) and returns
+  // unless they are its own.
+  for (int G : {1, 2, 4}) {
+    if (G > 1 && !grp) break;
+    const std::string kn = name + (G > 1 ? "_l" + std::to_string(G) : "");
+    o << "extern \"C\" __global__ __launch_bounds__(64) void " << kn
+      << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
+         "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
+         "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
+      << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap") << ") {\n"
+         "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
+    if (grp)
+      o << "  const int live = __builtin_amdgcn_readfirstlane(*count);\n"
+           "  const int pick = (gq >= 4 && live <= 16 * cap) ? 4 : ((gq >= 2 && live <= 32 * cap) ? 2 : 1);\n"
+           "  if (pick != " << G << ") return;\n";
+    if (gated)
+      o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
+           "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
+           "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+    o << "  __builtin_amdgcn_s_setprio(3);\n";
+    if (G == 1) o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
+    else o << "  sim_g" << G << "(A, blockIdx.x);\n}\n";
+  }
   return o.str();
 }
 
 std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated) { return gen_single(P, mixed, gated, false); }
 
 std::string gen_sim_struct_source(const ProblemDev& P, bool mixed, bool gated) { return gen_single(P, mixed, gated, true); }
+
+// Structural bodies of one structure: sim<tag>(A, Pp, bx) with one lane per candidate and, for
+// the float form, sim<tag>_g2 / _g4 with 2 and 4 lanes.
+static void gen_struct_bodies(std::ostringstream& o, const ProblemDev& P, bool mixed, const std::string& tag) {
+  o << "static __device__ inline __attribute__((always_inline)) void sim" << tag
+    << "(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
+  gen_struct_body(o, P, mixed);
+  o << "}\n";
+  if (!mixed) return;
+  for (int G : {2, 4}) {
+    Gen g;
+    g.f32 = true;
+    gen_body_grp(g, P, G, true);
+    o << "static __device__ inline __attribute__((always_inline)) void sim" << tag << "_g" << G
+      << "(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n" << g.o.str() << "}\n";
+  }
+}
+
+// call of structure tag's body with G lanes per candidate
+static std::string struct_call(const std::string& tag, int G, const std::string& args) {
+  return "sim" + tag + (G > 1 ? "_g" + std::to_string(G) : std::string()) + "(" + args + ");";
+}
 
 std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
   Gen g;
@@ -597,21 +664,24 @@ std::string gen_sim_batch_source(const ProblemDev& P, bool mixed) {
   o << "// generated: closed-loop simulation for every problem of one structure, ns=" << P.ns << " nd=" << P.nd
     << " nw=" << P.nw << (mixed ? " float" : " double") << "\n";
   o << kSimArgs;
-  o << "static __device__ inline __attribute__((always_inline)) void sim_one(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
-  gen_struct_body(o, P, mixed);
-  o << "}\n";
-  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
-    << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
-       "    const char* const* __restrict__ probs, int J, int total) {\n"
-       "  __builtin_amdgcn_s_setprio(3);\n"
-       "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
-       "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
-       "    int lo = 0, hi = J - 1;\n"
-       "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
-       "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
-       "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
-       "    sim_one(args[p], probs[p], lb);\n"
-       "  }\n}\n";
+  gen_struct_bodies(o, P, mixed, "_one");
+  // one kernel per lane count (NAME, NAME_l2, NAME_l4: each with its own register allocation); the
+  // launch's block table has 64 / lanes candidates per block
+  for (int G : {1, 2, 4}) {
+    if (G > 1 && !mixed) break;
+    o << "extern \"C\" __global__ __launch_bounds__(64) void " << name << (G > 1 ? "_l" + std::to_string(G) : std::string())
+      << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
+         "    const char* const* __restrict__ probs, int J, int total) {\n"
+         "  __builtin_amdgcn_s_setprio(3);\n"
+         "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
+         "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
+         "    int lo = 0, hi = J - 1;\n"
+         "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
+         "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
+         "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
+         "    " << struct_call("_one", G, "args[p], probs[p], lb") << "\n"
+         "  }\n}\n";
+  }
   return o.str();
 }
 
@@ -627,24 +697,25 @@ std::string gen_sim_batch_fused_source(const std::vector<const ProblemDev*>& rep
   o << kSimArgs;
   for (size_t k = 0; k < reps.size(); k++) {
     o << "// structure " << k << ": ns=" << reps[k]->ns << " nd=" << reps[k]->nd << " nw=" << reps[k]->nw << "\n";
-    o << "static __device__ inline __attribute__((always_inline)) void sim_s" << k
-      << "(const SimArgs& A, const char* __restrict__ Pp, int bx) {\n";
-    gen_struct_body(o, *reps[k], mixed);
-    o << "}\n";
+    gen_struct_bodies(o, *reps[k], mixed, "_s" + std::to_string(k));
   }
-  o << "extern \"C\" __global__ __launch_bounds__(64) void " << name
-    << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
-       "    const char* const* __restrict__ probs, const int* __restrict__ sid, int J, int total) {\n"
-       "  __builtin_amdgcn_s_setprio(3);\n"
-       "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
-       "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
-       "    int lo = 0, hi = J - 1;\n"
-       "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
-       "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
-       "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
-       "    switch (__builtin_amdgcn_readfirstlane(sid[lo])) {\n";
-  for (size_t k = 0; k < reps.size(); k++) o << "      case " << k << ": sim_s" << k << "(args[p], probs[p], lb); break;\n";
-  o << "    }\n  }\n}\n";
+  for (int G : {1, 2, 4}) {
+    if (G > 1 && !mixed) break;
+    o << "extern \"C\" __global__ __launch_bounds__(64) void " << name << (G > 1 ? "_l" + std::to_string(G) : std::string())
+      << "(const int* __restrict__ start, const int* __restrict__ pid, const SimArgs* __restrict__ args,\n"
+         "    const char* const* __restrict__ probs, const int* __restrict__ sid, int J, int total) {\n"
+         "  __builtin_amdgcn_s_setprio(3);\n"
+         "  for (int b = blockIdx.x; b < total; b += gridDim.x) {\n"
+         "    const int bu = __builtin_amdgcn_readfirstlane(b);\n"
+         "    int lo = 0, hi = J - 1;\n"
+         "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (start[mid] <= bu) lo = mid; else hi = mid - 1; }\n"
+         "    const int p = __builtin_amdgcn_readfirstlane(pid[lo]);\n"
+         "    const int lb = bu - __builtin_amdgcn_readfirstlane(start[lo]);\n"
+         "    switch (__builtin_amdgcn_readfirstlane(sid[lo])) {\n";
+    for (size_t k = 0; k < reps.size(); k++)
+      o << "      case " << k << ": " << struct_call("_s" + std::to_string(k), G, "args[p], probs[p], lb") << " break;\n";
+    o << "    }\n  }\n}\n";
+  }
   return o.str();
 }
 
@@ -652,6 +723,7 @@ std::string gen_sim_batch_fused_source(const std::vector<const ProblemDev*>& rep
 struct JitEntry {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
+  hipFunction_t fl[2] = {nullptr, nullptr};  // NAME_l2, NAME_l4 when the module has them
 };
 
 // Compiled modules by source text.  Entries added as evictable (the literal kernels, one per
@@ -696,13 +768,17 @@ long long jit_evictions(JitCache* cache) {
   return cache->evictions;
 }
 
-bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string& err, bool evictable) {
+bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string& err, bool evictable, void** fn_lanes) {
   std::lock_guard<std::mutex> lk(cache->mu);
   auto it = cache->entries.find(src);
   if (it != cache->entries.end()) {
     auto li = std::find(cache->lru.begin(), cache->lru.end(), src);
     if (li != cache->lru.end()) cache->lru.splice(cache->lru.begin(), cache->lru, li);
     *fn_out = (void*)it->second.fn;
+    if (fn_lanes) {
+      fn_lanes[0] = (void*)it->second.fl[0];
+      fn_lanes[1] = (void*)it->second.fl[1];
+    }
     return true;
   }
   hiprtcProgram prog;
@@ -739,6 +815,19 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
     err = std::string("hipModuleGetFunction(") + kname + ") failed";
     hipModuleUnload(e.mod);
     return false;
+  }
+  // lane-group kernels of the same module, where the source has them
+  for (int k = 0; k < 2; k++) {
+    const std::string ln = kname + (k ? "_l4" : "_l2");
+    if (src.find("void " + ln + "(") != std::string::npos && hipModuleGetFunction(&e.fl[k], e.mod, ln.c_str()) != hipSuccess) {
+      err = std::string("hipModuleGetFunction(") + ln + ") failed";
+      hipModuleUnload(e.mod);
+      return false;
+    }
+  }
+  if (fn_lanes) {
+    fn_lanes[0] = (void*)e.fl[0];
+    fn_lanes[1] = (void*)e.fl[1];
   }
   cache->entries[src] = e;
   cache->compiles++;

@@ -20,7 +20,9 @@ bool jit_enabled();
 std::string gen_sim_source(const ProblemDev& P, bool mixed, bool gated = false);
 // Compiled kernel of src (compiled on first use).  evictable: the entry may be unloaded once more
 // recent evictable entries exceed the cache's capacity (after a device drain).
-bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err, bool evictable = false);
+// fn_lanes (optional, 2 entries): the module's NAME_l2 and NAME_l4 kernels, or NULL where it has none.
+bool jit_get(JitCache* cache, const std::string& src, void** fn, std::string& err, bool evictable = false,
+             void** fn_lanes = nullptr);
 bool jit_cached(JitCache* cache, const std::string& src);
 long long jit_compiles(JitCache* cache);   // hipRTC compilations done by this cache
 long long jit_evictions(JitCache* cache);  // entries unloaded
@@ -57,6 +59,8 @@ struct SimArgs {
   int n;
   int pad;
 };
+// The float module also holds NAME_l2 and NAME_l4, the 2- and 4-lane forms (a launch's block table
+// has 64 / lanes candidates per block).
 std::string gen_sim_batch_source(const ProblemDev& P, bool mixed);
 std::string sim_structure_key(const ProblemDev& P, bool mixed);
 bool jit_launch_sim_batch(void* fn, int grid, hipStream_t st, const int* start, const int* pid, const SimArgs* args,
