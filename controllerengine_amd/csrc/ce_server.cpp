@@ -134,9 +134,23 @@ struct ce_server {
   Clock::time_point t_first, t_last;
   bool any_done = false, any_submit = false;
   std::atomic<bool> stop{false};
+  bool wake = false;        // a slot's work completed (set by the stream callback, under mu)
+  long long submits = 0;    // jobs submitted so far (under mu)
   std::thread thr;
   std::string fatal;
 };
+
+// Runs on a HIP runtime thread once a slot's batch, output copies and done event have completed:
+// wakes the scheduler (no HIP call here).  The scheduler never holds mu while it waits on a
+// stream, so this lock cannot deadlock.
+static void slot_done_cb(void* arg) {
+  ce_server* S = static_cast<ce_server*>(arg);
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    S->wake = true;
+  }
+  S->cv.notify_all();
+}
 
 static void put_err_s(char* err, int errlen, const std::string& m) {
   if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", m.c_str());
@@ -178,6 +192,7 @@ static void launch_slot(ce_server* S, Slot& s) {
       hipMemcpyAsync(ho64, dout64, ni * 8, hipMemcpyDeviceToHost, s.st) != hipSuccess)
     throw std::runtime_error("output copy failed");
   if (hipEventRecord(s.done, s.st) != hipSuccess) throw std::runtime_error("event record failed");
+  if (hipLaunchHostFunc(s.st, slot_done_cb, S) != hipSuccess) throw std::runtime_error("host callback failed");
 }
 
 static void finish_slot(ce_server* S, Slot& s) {
@@ -213,6 +228,8 @@ static void scheduler(ce_server* S) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (!S->stop) {
     bool progressed = false;
+    S->wake = false;
+    const long long seen = S->submits;
     for (auto& sp : S->slots) {
       Slot& s = *sp;
       if (s.busy && hipEventQuery(s.done) == hipSuccess) {
@@ -254,9 +271,9 @@ static void scheduler(ce_server* S) {
       } catch (const std::exception& e) {
         fail = e.what();
       }
+      if (!fail.empty()) hipStreamSynchronize(s.st);  // without the lock: a pending callback takes it
       lk.lock();
       if (!fail.empty()) {
-        hipStreamSynchronize(s.st);
         for (auto& j : s.jobs) {
           j->state = FAILED;
           j->error = fail;
@@ -269,7 +286,9 @@ static void scheduler(ce_server* S) {
       }
       progressed = true;
     }
-    if (!progressed) S->cv.wait_for(lk, std::chrono::microseconds(100));
+    // sleep until a submission, a completion callback or stop (the timeout only bounds a missed wake)
+    if (!progressed)
+      S->cv.wait_for(lk, std::chrono::milliseconds(5), [S, seen] { return S->stop || S->wake || S->submits != seen; });
   }
 }
 
@@ -351,6 +370,7 @@ extern "C" CE_EXPORT long long ce_server_submit(ce_server* S, const ce_config* c
     }
     S->jobs[j->id] = j;
     S->queue.push_back(j);
+    S->submits++;
   }
   S->cv.notify_all();
   return j->id;
