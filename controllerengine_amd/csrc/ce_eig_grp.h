@@ -59,29 +59,72 @@ __device__ __forceinline__ double grp_max_d(double v) {
   return v;
 }
 
+// The value v holds in lane SRC of the caller's group, in every lane of the group (all lanes of
+// the group active).  Groups of 16 lanes are the rows of the wave and groups of 4 its quads, so a
+// DPP row broadcast / quad permute moves the two halves without LDS; other group sizes permute
+// through the LDS crossbar.
+template <int G, int SRC>
+__device__ __forceinline__ double grp_bcast(double v) {
+  static_assert(SRC < G, "source lane outside the group");
+  if constexpr (G == 16 || G == 4) {
+    constexpr int ctrl = (G == 16) ? (0x150 + SRC) : (SRC * 0x55);  // row_newbcast:SRC / quad_perm [SRC x4]
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, ctrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  } else {
+    return __shfl(v, (int)(threadIdx.x & ~(unsigned)(G - 1)) + SRC);
+  }
+}
+
+// IEEE square root.  For x in [2^-700, 2^700] this is the sequence the compiler emits for sqrt on
+// gfx950 (hardware reciprocal square root, then Goldschmidt and two Newton corrections), without
+// its range scaling and special-value selects, which leave such x unchanged; other x take the full
+// sqrt.  The same bits as sqrt for every x (tests/test_gpu_eig_math.py).
+__device__ __forceinline__ double grp_sqrt(double x) {
+  if (x >= 0x1p-700 && x <= 0x1p700) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  }
+  return cedm::sqrt_(x);
+}
+
 struct GHh {
   double e0, e1, tau, beta;
 };
 
-__device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int len, int* event) {
+// Householder reflector of (c0, v1[, v2]) (Eigen's makeHouseholderInPlace).  Every lane of the
+// group evaluates the norm and beta; the quotients e0 = v1 / den, e1 = v2 / den and
+// tau = (beta - c0) / beta = -(den / beta) (beta - c0 rounds to exactly -den) are IEEE divisions on
+// lanes 0, 1 and 2 of the group, broadcast to the others: the roundings of evaluating all three in
+// every lane, at a third of the divisions.  Without branches: when the tail is negligible (tau = 0,
+// beta = c0) the quotients are evaluated and ignored, since callers apply the reflector only when
+// tau != 0.
+template <int G>
+__device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int len, int l, int* event) {
+  static_assert(G >= 3, "the quotients need three lanes");
   GHh h;
   double tail = cedm::fma_(v1, v1, 0.0);
   if (len == 3) tail = cedm::fma_(v2, v2, tail);
   if (event && c0 == 0.0 && tail > DBL_MIN) *event = 1;
-  if (tail <= DBL_MIN) {
-    h.tau = 0.0;
-    h.beta = c0;
-    h.e0 = 0.0;
-    h.e1 = 0.0;
-  } else {
-    double beta = cedm::sqrt_(cedm::fma_(c0, c0, tail));
-    if (c0 >= 0.0) beta = -beta;
-    const double den = c0 - beta;
-    h.e0 = v1 / den;
-    h.e1 = (len == 3) ? v2 / den : 0.0;
-    h.tau = (beta - c0) / beta;
-    h.beta = beta;
-  }
+  const bool flat = tail <= DBL_MIN;
+  double beta = grp_sqrt(cedm::fma_(c0, c0, tail));
+  if (c0 >= 0.0) beta = -beta;
+  const double den = c0 - beta;
+  const double num = (l == 0) ? v1 : ((l == 1) ? v2 : den);
+  const double q = num / ((l == 2) ? beta : den);
+  h.e0 = grp_bcast<G, 0>(q);
+  h.e1 = (len == 3) ? grp_bcast<G, 1>(q) : 0.0;
+  h.tau = flat ? 0.0 : -grp_bcast<G, 2>(q);
+  h.beta = flat ? c0 : beta;
   return h;
 }
 
@@ -126,7 +169,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
       tau = 0.0;
       beta = c0;
     } else {
-      beta = cedm::sqrt_(cedm::fma_(c0, c0, tail));
+      beta = grp_sqrt(cedm::fma_(c0, c0, tail));
       if (c0 >= 0.0) beta = -beta;
       den = c0 - beta;
       tau = (beta - c0) / beta;
@@ -216,7 +259,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
         }
         grp_sync();
         if (q >= 0.0) {
-          const double z = cedm::sqrt_(cedm::fabs_(q));
+          const double z = grp_sqrt(cedm::fabs_(q));
           const double gp = (p >= 0.0) ? p + z : p - z;
           double c, s;
           if (gq == 0.0) {
@@ -227,13 +270,13 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             s = gq < 0.0 ? 1.0 : -1.0;
           } else if (cedm::fabs_(gp) > cedm::fabs_(gq)) {
             const double t = gq / gp;
-            double u = cedm::sqrt_(1.0 + t * t);
+            double u = grp_sqrt(1.0 + t * t);
             if (gp < 0.0) u = -u;
             c = 1.0 / u;
             s = -t * c;
           } else {
             const double t = gp / gq;
-            double u = cedm::sqrt_(1.0 + t * t);
+            double u = grp_sqrt(1.0 + t * t);
             if (gq < 0.0) u = -u;
             s = -1.0 / u;
             c = -t * s;
@@ -274,7 +317,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           double s = (sh1 - sh0) / 2.0;
           s = s * s + sh2;
           if (s > 0.0) {
-            s = cedm::sqrt_(s);
+            s = grp_sqrt(s);
             if (sh1 < sh0) s = -s;
             s = s + (sh1 - sh0) / 2.0;
             s = sh0 - sh2 / s;
@@ -321,12 +364,10 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
           v1 = T[(im + 1) * S + im + 1] - tmm - r - s;
           v2 = T[(im + 2) * S + im + 1];
         }
+        double w0 = v0, w1 = v1, w2 = v2;  // the vector of step k: column k - 1 below row k - 1 after the first
         for (int k = im; k <= iu - 2; k++) {
           const bool first = (k == im);
-          double w0, w1, w2;
-          if (first) { w0 = v0; w1 = v1; w2 = v2; }
-          else { w0 = T[k * S + k - 1]; w1 = T[(k + 1) * S + k - 1]; w2 = T[(k + 2) * S + k - 1]; }
-          const GHh h = grp_make_hh(w0, w1, w2, 3, event);
+          const GHh h = grp_make_hh<G>(w0, w1, w2, 3, l, event);
           if (h.beta != 0.0) {
             grp_sync();
             if (l == 0) {
@@ -357,9 +398,14 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             }
             grp_sync();
           }
+          if (k < iu - 2) {
+            w0 = T[(k + 1) * S + k];
+            w1 = T[(k + 2) * S + k];
+            w2 = T[(k + 3) * S + k];
+          }
         }
         {
-          const GHh h = grp_make_hh(T[(iu - 1) * S + iu - 2], T[iu * S + iu - 2], 0.0, 2, event);
+          const GHh h = grp_make_hh<G>(T[(iu - 1) * S + iu - 2], T[iu * S + iu - 2], 0.0, 2, l, event);
           if (h.beta != 0.0) {
             grp_sync();
             if (l == 0) T[(iu - 1) * S + iu - 2] = h.beta;
@@ -412,7 +458,7 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
       t0 = t0 / maxval;
       t1 = t1 / maxval;
       const double p0 = p / maxval;
-      const double z = maxval * cedm::sqrt_(cedm::fabs_(p0 * p0 + t0 * t1));
+      const double z = maxval * grp_sqrt(cedm::fabs_(p0 * p0 + t0 * t1));
       const double w = T[(i + 1) * S + i + 1] + p;
       if (l == 0) { WR[i] = w; WI[i] = z; WR[i + 1] = w; WI[i + 1] = -z; }
       if (!(cedm::isfinite_(w) && cedm::isfinite_(z))) { rc = 2; break; }

@@ -576,6 +576,7 @@ extern "C" CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* 
       case 4: out[i] = cedm::atan2(x[i], y[i]); break;
       case 5: out[i] = cedm::hypot(x[i], y[i]); break;
       case 6: out[i] = cedm::log10(x[i]); break;
+      case 7: out[i] = cedm::sqrt_(x[i]); break;
       default: return 1;
     }
   }

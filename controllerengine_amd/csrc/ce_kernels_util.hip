@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ce_common.h"
+#include "ce_eig_grp.h"
 
 namespace ce {
 
@@ -28,6 +29,7 @@ __global__ __launch_bounds__(256) void k_detmath(int fn, const double* x, const 
     case 4: r = cedm::atan2(x[i], y[i]); break;
     case 5: r = cedm::hypot(x[i], y[i]); break;
     case 6: r = cedm::log10(x[i]); break;
+    case 7: r = grp_sqrt(x[i]); break;  // the eigensolver's square root
     default: r = cedm::qnan(); break;
   }
   out[i] = r;

@@ -270,7 +270,8 @@ CE_EXPORT int ce_rank_noalloc(const ce_config* cfg, const float* d_info, int n, 
                               uint8_t* d_pass, int32_t* d_npass, void* stream, char* err, int errlen);
 
 /* Host-side helpers exported for tests: deterministic double math (fn: 0 exp, 1 log, 2 sin, 3 cos,
- * 4 atan2, 5 hypot, 6 log10) on the CPU and on the device. */
+ * 4 atan2, 5 hypot, 6 log10, 7 sqrt) on the CPU and on the device (7: the device eigensolver's
+ * square root against the host's IEEE sqrt). */
 CE_EXPORT int ce_detmath_host(int fn, const double* x, const double* y, double* out, int n);
 CE_EXPORT int ce_detmath_device(int fn, const double* d_x, const double* d_y, double* d_out, int n, void* stream);
 
