@@ -12,6 +12,13 @@
 //    updates column j in lane j, a right reflector row i in lane i), and each element sees exactly
 //    the operations, in the same order, as in the per-lane solver.
 //
+// Only eigenvalues are computed, so the QR phase leaves out what the sequential solver does for
+// the Schur vectors' sake but never reads again: left reflectors stop at column iu (the solver's
+// active window ends there and iu never grows back, so columns beyond it are read by no later
+// step), and the 2 x 2 standardisation rotates only its own block.  Right reflectors keep every
+// row from 0: the window's top il can move back up (its deflation test is relative to a diagonal
+// entry the iteration changes), after which rows above the old window are read again.
+//
 // The result is therefore bit-identical to the sequential solver.  Groups of one wave may diverge
 // from each other; a group never waits on another.  Phases that read what other lanes of the group
 // wrote are separated by grp_sync(): a wavefront-scope fence, which keeps the compiler from moving
@@ -130,13 +137,14 @@ __device__ __forceinline__ GHh grp_make_hh(double c0, double v1, double v2, int 
 
 // Eigenvalues of the n x n matrix T (LDS, row stride S = G R + 1, destroyed) into WR/WI (LDS, n
 // entries), n <= G R.  Lane l of the group owns the rows / columns l, l + G, ..., l + (R-1) G.
-// V: LDS scratch of at least n doubles.  Returns (in every lane) 0 = success, 1 = no convergence
+// V: LDS scratch of 3 G R doubles (WR and WI may lie inside it: they are written only at the
+// read-out).  Returns (in every lane) 0 = success, 1 = no convergence
 // within 40 n iterations, 2 = non-finite eigenvalue.
 #define CE_LANES(j) _Pragma("unroll") for (int q_ = 0, j = l; q_ < R; q_++, j += G)
 template <int G, int R = 1>
 __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double* WI, int n, int l, int sign_block = -1,
                                        int* event = nullptr) {
-  constexpr int S = G * R + 1;
+  constexpr int N = G * R, S = N + 1;
   // largest |a_ij| (NaN entries never win, as in the sequential fold): column maxima per lane,
   // then the group maximum
   double scale = 0.0;
@@ -225,81 +233,91 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
   consider_zero = (consider_zero < DBL_MIN) ? DBL_MIN : consider_zero;
   if (norm != 0.0) {
     while (iu >= 0) {
-      // findSmallSubdiagEntry: the largest il <= iu whose subdiagonal entry is negligible (0 when
-      // none is).  Every lane tests its own rows; the group maximum is the row the downward scan
-      // stops at.
+      // Deflations run in a loop of their own until the window needs an iteration, so every pass
+      // of the outer loop is one Francis iteration in every group that is not finished: groups of
+      // a wave that deflate at different passes do not wait through each other's iterations.
       int il;
-      {
-        int best = 0;
-        CE_LANES(r) if (r >= 1 && r <= iu) {
-          double s = cedm::fabs_(T[(r - 1) * S + r - 1]) + cedm::fabs_(T[r * S + r]);
-          s = s * eps;
-          s = (s < consider_zero) ? consider_zero : s;
-          if (cedm::fabs_(T[r * S + r - 1]) <= s && r > best) best = r;
+      for (;;) {
+        // findSmallSubdiagEntry: the largest il <= iu whose subdiagonal entry is negligible (0 when
+        // none is).  Every lane tests its own rows; the group maximum is the row the downward scan
+        // stops at.
+        {
+          int best = 0;
+          CE_LANES(r) if (r >= 1 && r <= iu) {
+            double s = cedm::fabs_(T[(r - 1) * S + r - 1]) + cedm::fabs_(T[r * S + r]);
+            s = s * eps;
+            s = (s < consider_zero) ? consider_zero : s;
+            if (cedm::fabs_(T[r * S + r - 1]) <= s && r > best) best = r;
+          }
+          il = grp_max_i<G>(best);
         }
-        il = grp_max_i<G>(best);
+        if (il == iu) {
+          grp_sync();
+          if (l == 0) {
+            T[iu * S + iu] = T[iu * S + iu] + exshift;
+            if (iu > 0) T[iu * S + iu - 1] = 0.0;
+          }
+          grp_sync();
+          iu--;
+          iter = 0;
+        } else if (il == iu - 1) {
+          const double p = 0.5 * (T[(iu - 1) * S + iu - 1] - T[iu * S + iu]);
+          const double q = p * p + T[iu * S + iu - 1] * T[(iu - 1) * S + iu];
+          const double gq = T[iu * S + iu - 1];
+          grp_sync();
+          if (l == 0) {
+            T[iu * S + iu] = T[iu * S + iu] + exshift;
+            T[(iu - 1) * S + iu - 1] = T[(iu - 1) * S + iu - 1] + exshift;
+          }
+          grp_sync();
+          if (q >= 0.0) {
+            const double z = grp_sqrt(cedm::fabs_(q));
+            const double gp = (p >= 0.0) ? p + z : p - z;
+            double c, s;
+            if (gq == 0.0) {
+              c = gp < 0.0 ? -1.0 : 1.0;
+              s = 0.0;
+            } else if (gp == 0.0) {
+              c = 0.0;
+              s = gq < 0.0 ? 1.0 : -1.0;
+            } else if (cedm::fabs_(gp) > cedm::fabs_(gq)) {
+              const double t = gq / gp;
+              double u = grp_sqrt(1.0 + t * t);
+              if (gp < 0.0) u = -u;
+              c = 1.0 / u;
+              s = -t * c;
+            } else {
+              const double t = gp / gq;
+              double u = grp_sqrt(1.0 + t * t);
+              if (gq < 0.0) u = -u;
+              s = -1.0 / u;
+              c = -t * s;
+            }
+            CE_LANES(j) if (j >= iu - 1 && j <= iu) {
+              const double x = T[(iu - 1) * S + j], y = T[iu * S + j];
+              T[(iu - 1) * S + j] = c * x - s * y;
+              T[iu * S + j] = s * x + c * y;
+            }
+            grp_sync();
+            CE_LANES(r) if (r >= iu - 1 && r <= iu) {
+              const double x = T[r * S + iu - 1], y = T[r * S + iu];
+              T[r * S + iu - 1] = c * x - s * y;
+              T[r * S + iu] = s * x + c * y;
+            }
+            grp_sync();
+            if (l == 0) T[iu * S + iu - 1] = 0.0;
+          }
+          if (l == 0 && iu > 1) T[(iu - 1) * S + iu - 2] = 0.0;
+          grp_sync();
+          iu -= 2;
+          iter = 0;
+        } else {
+          break;
+        }
+        if (iu < 0) break;
       }
-      if (il == iu) {
-        grp_sync();
-        if (l == 0) {
-          T[iu * S + iu] = T[iu * S + iu] + exshift;
-          if (iu > 0) T[iu * S + iu - 1] = 0.0;
-        }
-        grp_sync();
-        iu--;
-        iter = 0;
-      } else if (il == iu - 1) {
-        const double p = 0.5 * (T[(iu - 1) * S + iu - 1] - T[iu * S + iu]);
-        const double q = p * p + T[iu * S + iu - 1] * T[(iu - 1) * S + iu];
-        const double gq = T[iu * S + iu - 1];
-        grp_sync();
-        if (l == 0) {
-          T[iu * S + iu] = T[iu * S + iu] + exshift;
-          T[(iu - 1) * S + iu - 1] = T[(iu - 1) * S + iu - 1] + exshift;
-        }
-        grp_sync();
-        if (q >= 0.0) {
-          const double z = grp_sqrt(cedm::fabs_(q));
-          const double gp = (p >= 0.0) ? p + z : p - z;
-          double c, s;
-          if (gq == 0.0) {
-            c = gp < 0.0 ? -1.0 : 1.0;
-            s = 0.0;
-          } else if (gp == 0.0) {
-            c = 0.0;
-            s = gq < 0.0 ? 1.0 : -1.0;
-          } else if (cedm::fabs_(gp) > cedm::fabs_(gq)) {
-            const double t = gq / gp;
-            double u = grp_sqrt(1.0 + t * t);
-            if (gp < 0.0) u = -u;
-            c = 1.0 / u;
-            s = -t * c;
-          } else {
-            const double t = gp / gq;
-            double u = grp_sqrt(1.0 + t * t);
-            if (gq < 0.0) u = -u;
-            s = -1.0 / u;
-            c = -t * s;
-          }
-          CE_LANES(j) if (j >= iu - 1 && j < n) {
-            const double x = T[(iu - 1) * S + j], y = T[iu * S + j];
-            T[(iu - 1) * S + j] = c * x - s * y;
-            T[iu * S + j] = s * x + c * y;
-          }
-          grp_sync();
-          CE_LANES(r) if (r <= iu) {
-            const double x = T[r * S + iu - 1], y = T[r * S + iu];
-            T[r * S + iu - 1] = c * x - s * y;
-            T[r * S + iu] = s * x + c * y;
-          }
-          grp_sync();
-          if (l == 0) T[iu * S + iu - 1] = 0.0;
-        }
-        if (l == 0 && iu > 1) T[(iu - 1) * S + iu - 2] = 0.0;
-        grp_sync();
-        iu -= 2;
-        iter = 0;
-      } else {
+      if (iu < 0) break;
+      {
         double sh0 = T[iu * S + iu];
         double sh1 = T[(iu - 1) * S + iu - 1];
         double sh2 = T[iu * S + iu - 1] * T[(iu - 1) * S + iu];
@@ -367,6 +385,14 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
         double w0 = v0, w1 = v1, w2 = v2;  // the vector of step k: column k - 1 below row k - 1 after the first
         for (int k = im; k <= iu - 2; k++) {
           const bool first = (k == im);
+          // Rows k..k+2 of the lane's window columns, loaded before the reflector is formed (nothing
+          // writes them before the left update).
+          double a0[R], a1[R], a2[R];
+          CE_LANES(j) if (j >= k && j <= iu) {
+            a0[q_] = T[k * S + j];
+            a1[q_] = T[(k + 1) * S + j];
+            a2[q_] = T[(k + 2) * S + j];
+          }
           const GHh h = grp_make_hh<G>(w0, w1, w2, 3, l, event);
           if (h.beta != 0.0) {
             grp_sync();
@@ -375,13 +401,14 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
               else if (!first) T[k * S + k - 1] = h.beta;
             }
             if (h.tau != 0.0) {
-              CE_LANES(j) if (j >= k && j < n) {  // left: rows k..k+2 of column j
-                double tmp = cedm::fma_(h.e0, T[(k + 1) * S + j], 0.0);
-                tmp = cedm::fma_(h.e1, T[(k + 2) * S + j], tmp);
-                tmp = tmp + T[k * S + j];
-                T[k * S + j] = T[k * S + j] - h.tau * tmp;
-                T[(k + 1) * S + j] = T[(k + 1) * S + j] - (h.tau * h.e0) * tmp;
-                T[(k + 2) * S + j] = T[(k + 2) * S + j] - (h.tau * h.e1) * tmp;
+              const double te0 = h.tau * h.e0, te1 = h.tau * h.e1;
+              CE_LANES(j) if (j >= k && j <= iu) {  // left: rows k..k+2 of column j
+                double tmp = cedm::fma_(h.e0, a1[q_], 0.0);
+                tmp = cedm::fma_(h.e1, a2[q_], tmp);
+                tmp = tmp + a0[q_];
+                T[k * S + j] = a0[q_] - h.tau * tmp;
+                T[(k + 1) * S + j] = a1[q_] - te0 * tmp;
+                T[(k + 2) * S + j] = a2[q_] - te1 * tmp;
               }
               grp_sync();
               const int r1 = ((iu < k + 3) ? iu : k + 3) + 1;
@@ -410,14 +437,14 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
             grp_sync();
             if (l == 0) T[(iu - 1) * S + iu - 2] = h.beta;
             if (h.tau != 0.0) {
-              CE_LANES(j) if (j >= iu - 1 && j < n) {
+              CE_LANES(j) if (j >= iu - 1 && j <= iu) {
                 double tmp = cedm::fma_(h.e0, T[iu * S + j], 0.0);
                 tmp = tmp + T[(iu - 1) * S + j];
                 T[(iu - 1) * S + j] = T[(iu - 1) * S + j] - h.tau * tmp;
                 T[iu * S + j] = T[iu * S + j] - (h.tau * h.e0) * tmp;
               }
               grp_sync();
-              CE_LANES(r) if (r < iu + 1) {
+              CE_LANES(r) if (r <= iu) {
                 double* row = T + r * S + iu - 1;
                 double tmp = cedm::fma_(row[1], h.e0, 0.0);
                 tmp = tmp + row[0];
@@ -438,8 +465,8 @@ __device__ __forceinline__ int eig_grp(double* T, double* V, double* WR, double*
     }
   }
   grp_sync();
-  CE_LANES(j) if (j < n)
-    for (int i = 0; i < n; i++) T[i * S + j] = T[i * S + j] * scale;
+  CE_LANES(j) if (j < n)  // the read-out uses the diagonal, sub- and superdiagonal
+    for (int i = (j > 0 ? j - 1 : 0); i <= j + 1 && i < n; i++) T[i * S + j] = T[i * S + j] * scale;
   grp_sync();
   int rc = 0;
   int i = 0;
