@@ -150,9 +150,12 @@ static void gen_phase_loop(std::ostringstream& o) {
 // into a named register value (kcN); otherwise they are read every step through Pq, which an empty
 // asm statement redefines (in scalar registers) every step, so the loads stay scalar loads from the
 // scalar cache instead of being hoisted into vector registers for the whole loop (for structures
-// with more constants than registers to spare).  *n_hoisted: 32-bit words of hoisted constants.
+// with more constants than registers to spare).  With vbudget > 0 (float form), the first vbudget
+// float constants the step reads are copied into vector registers once before the loop (an opaque
+// v_mov_b32, so they stay there) and the rest are read every step through Pq.  *n_hoisted: 32-bit
+// words of hoisted constants.
 static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, bool hoist = false,
-                     int* n_hoisted = nullptr) {
+                     int* n_hoisted = nullptr, int vbudget = 0) {
   const int ns = P.ns, nd = P.nd, nw = P.nw;
   std::ostringstream& o = g.o;
   const std::string T = g.T();
@@ -167,6 +170,17 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
   };
   auto ld = [&](const char* ty, size_t off) {
     if (!in_loop) return ld0(ty, off, "Pp");
+    if (vbudget > 0) {
+      auto it = hname.find(off);
+      if (it != hname.end()) return it->second;
+      if (std::string(ty) != "float" || hwords + 1 > vbudget) return ld0(ty, off, "Pq");
+      const std::string nm = "kv" + std::to_string(hname.size());
+      hdecl << "  float " << nm << ";\n  asm volatile(\"v_mov_b32 %0, %1\" : \"=v\"(" << nm << ") : \"s\"("
+            << ld0(ty, off, "Pp") << "));\n";
+      hwords += 1;
+      hname[off] = nm;
+      return nm;
+    }
     if (!hoist) return ld0(ty, off, "Pq");
     auto it = hname.find(off);
     if (it != hname.end()) return it->second;
@@ -205,14 +219,16 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
   o << "  " << T << "* yb = (" << T << "*)A.ybuf + idx;\n";
   o << "  float* dy = A.dy;\n";
   o << "  const " << T << " amp = " << K(P.ref_amp, CE_OFF(ref_amp), CE_OFF(fref_amp)) << ";\n";
-  if (structural && !hoist) o << "  const char* Pq = Pp;\n";
+  const bool per_step = structural && (!hoist || vbudget > 0);  // some constants read every step
+  const bool hoisting = structural && (hoist || vbudget > 0);
+  if (per_step) o << "  const char* Pq = Pp;\n";
   const std::string hoist_mark = "  /*hoisted constants*/\n";
-  if (structural && hoist) o << hoist_mark;
+  if (hoisting) o << hoist_mark;
   // one time step; the phase flags (fit window stored, steady-state sum, reference still zero,
   // full-trajectory dump) are compile-time constants in the phase loops below, so a step carries
   // no per-step branches besides the loop's own
   o << "  auto step = [&](const int t, auto ST, auto SS, auto RZ, auto DY) __attribute__((always_inline)) {\n";
-  if (structural && !hoist)
+  if (per_step)
     o << "    asm volatile(\"\" : \"+s\"(Pq));\n"
          "    Pq = (const char*)(((unsigned long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long)Pq >> 32)) << 32) |\n"
          "                       (unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)(unsigned long)Pq));\n";
@@ -325,7 +341,7 @@ static void gen_body(Gen& g, const ProblemDev& P, bool mixed, bool structural, b
   }
   o << "  };\n";
   in_loop = false;
-  if (structural && hoist) {
+  if (hoisting) {
     std::string text = o.str();
     const size_t at = text.find(hoist_mark);
     text.replace(at, hoist_mark.size(), hdecl.str());
@@ -557,10 +573,25 @@ static const char* kSimArgs =
 //  * stores gate_gen to *gate when its last workgroup starts: by then the workgroups before it have
 //    been placed, and a bounded waiter on the second stream lets the kernels there start, on the
 //    SIMDs the simulation leaves free.
-// Body of a structural kernel: constants loaded from the descriptor, hoisted out of the time loop
-// when they fit a modest register budget (kHoistMaxWords 32-bit words), else read every step.
+// Body of a structural kernel: constants loaded from the descriptor.
+//  * Float form: up to kVgprHoistWords constants in vector registers, the rest read every step.
+//    On gfx950 a wave64 FP32 instruction issues in about two cycles, but two that both read a
+//    scalar register take about four, and almost every multiply-add of the step reads a plant
+//    constant: with the constants in scalar registers the step issues about 1.4 times slower (a
+//    half of the instructions reading scalar registers costs nothing).  The budget keeps the
+//    kernel near 256 registers, two waves per SIMD, which hold the SIMD's issue slots busy.
+//  * FP64 form (four cycles per instruction whatever its operands): hoisted into scalar registers
+//    when they fit kHoistMaxWords 32-bit words, else read every step.
 static void gen_struct_body(std::ostringstream& o, const ProblemDev& P, bool mixed) {
   constexpr int kHoistMaxWords = 64;
+  constexpr int kVgprHoistWords = 128;
+  if (mixed) {
+    Gen gv;
+    gv.f32 = true;
+    gen_body(gv, P, true, true, false, nullptr, kVgprHoistWords);
+    o << gv.o.str();
+    return;
+  }
   Gen gh;
   gh.f32 = mixed;
   int words = 0;
