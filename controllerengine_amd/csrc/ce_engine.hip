@@ -36,7 +36,7 @@ bool launch_sweep_mixed(int ns, int F, const ProblemDev* P, const StageBufs& B, 
 bool mixed_sweep_fits(int F);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_compact(const ProblemDev* P, const StageBufs& B, hipStream_t st);
-void launch_gate_wait(const int* gate, int gen, hipStream_t st);
+void launch_gate_wait(const int* gate, int gen, int limit, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 void launch_detmath(int fn, const double* x, const double* y, double* out, int n, hipStream_t st);
 bool launch_sim_kalman(int ns, int nd, bool mixed, const ProblemDev* P, const float* gains, int n, double* y64,
@@ -306,7 +306,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   // stage 1: linearised closed loop, eigenvalues, modes, stability flags (FP64 in both modes)
   MarkerScope marks(E, st);
   HIPCHK(hipEventRecord(E->ev[0], st));
-  if (gated && !E->gate.p) {
+  if (fork && !E->gate.p) {
     ensure(E->gate, sizeof(int));
     launch_fill_i32((int*)E->gate.p, 0, 1, st);
   }
@@ -327,10 +327,16 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   if (fork) {
     HIPCHK(hipEventRecord(E->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
-    if (gated) {
-      E->gate_gen = (E->gate_gen >= (1 << 30)) ? 1 : E->gate_gen + 1;
-      launch_gate_wait((const int*)E->gate.p, E->gate_gen, E->side);
-    }
+    // The second stream starts when the simulation leaves room: the gated form signals once all its
+    // workgroups are placed (whole SIMDs stay free beside it); the ungated form (more waves than
+    // SIMDs) when its first working wave finishes, i.e. when SIMDs begin to empty.  Started with
+    // the simulation instead, the second stream's waves take issue slots from simulation waves
+    // still running (c3: step 17.9 -> 18.4 ms with the sweep beside the whole simulation).  The
+    // waits are bounded (512 polls at the start, 65536 at the end, a few ms of polling at most):
+    // a late signal only delays the second stream.  The interpreted kernel (no JIT) signals nothing
+    // and is not waited for.
+    E->gate_gen = (E->gate_gen >= (1 << 30)) ? 1 : E->gate_gen + 1;
+    if (fn) launch_gate_wait((const int*)E->gate.p, E->gate_gen, gated ? 512 : 65536, E->side);
     launch_eig_nom(H.ns, P, B, E->side);
     if (!launch_mark_refine(ns, F, P, B, E->side)) throw std::runtime_error("no sweep kernel for these dimensions");
     // the crossing scan needs only the sweep: it runs here, beside the fit, and leaves its minima
@@ -357,7 +363,8 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     if (st_form) {
       ok = gated ? jit_launch_sim_struct_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
                                                n, (int*)E->gate.p, E->gate_gen, P)
-                 : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P);
+                 : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P,
+                                         fork ? (int*)E->gate.p : nullptr, E->gate_gen);
     } else {
       ok = true;
       for (int G : {4, 2, 1}) {
@@ -366,7 +373,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
         ok = ok && (gated ? jit_launch_sim_gated(f, G * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
                                                  n, (int*)E->gate.p, E->gate_gen, gq, E->n_simd)
                           : jit_launch_sim(f, G * sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n,
-                                           gq, E->n_simd));
+                                           gq, E->n_simd, fork ? (int*)E->gate.p : nullptr, E->gate_gen));
       }
     }
     if (!ok) throw std::runtime_error("launch of the specialised simulation kernel failed");

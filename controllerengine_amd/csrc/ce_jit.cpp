@@ -643,7 +643,8 @@ static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool 
       << "(const float* __restrict__ gains, const int* __restrict__ flags,\n"
          "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
          "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
-      << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap") << ") {\n"
+      << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap")
+      << (gated ? "" : ", int* __restrict__ done, int done_gen") << ") {\n"
          "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
     if (grp)
       o << "  const int live = __builtin_amdgcn_readfirstlane(*count);\n"
@@ -654,8 +655,15 @@ static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool 
            "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
            "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
     o << "  __builtin_amdgcn_s_setprio(3);\n";
-    if (G == 1) o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n}\n";
-    else o << "  sim_g" << G << "(A, blockIdx.x);\n}\n";
+    if (!gated) o << "  const int cnt1 = __builtin_amdgcn_readfirstlane(*count);\n";
+    if (G == 1) o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n";
+    else o << "  sim_g" << G << "(A, blockIdx.x);\n";
+    // ungated: the first working wave to finish (or block 0 when none is live) signals done_gen
+    if (!gated)
+      o << "  if (done != nullptr && threadIdx.x == 0 && ((int)blockIdx.x * " << 64 / G
+        << " < cnt1 || (blockIdx.x == 0 && cnt1 == 0)))\n"
+           "    __hip_atomic_store(done, done_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+    o << "}\n";
   }
   return o.str();
 }
@@ -882,8 +890,9 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
 }
 
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gq, &cap};
+                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap, int* done,
+                    int done_gen) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gq, &cap, &done, &done_gen};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 
@@ -895,8 +904,9 @@ bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains
 }
 
 bool jit_launch_sim_struct(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &prob};
+                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob, int* done,
+                           int done_gen) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &prob, &done, &done_gen};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

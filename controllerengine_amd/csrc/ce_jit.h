@@ -30,8 +30,11 @@ long long jit_evictions(JitCache* cache);  // entries unloaded
 // kernel argument), so one compiled kernel serves every problem of the same dimensions and
 // instruction structure (sim_structure_key), e.g. the rounds of a serial-mode run.
 std::string gen_sim_struct_source(const ProblemDev& P, bool mixed, bool gated = false);
+// The ungated forms (here and jit_launch_sim) store done_gen to *done (when not NULL) as soon as
+// the first working wave has finished its time loop, or at once when no candidate is live.
 bool jit_launch_sim_struct(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob);
+                           const int* count, void* ybuf, double* info, float* dy, int n, const void* prob,
+                           int* done = nullptr, int done_gen = 0);
 bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                                  const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen,
                                  const void* prob);
@@ -39,7 +42,8 @@ bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float
 // candidate), cap: waves the device runs one per SIMD.  The float kernel runs the most lanes per
 // candidate, up to gq, whose live candidates' waves stay within cap; the FP64 kernel runs one.
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
-                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap);
+                    const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap, int* done = nullptr,
+                    int done_gen = 0);
 // the gated form: its last workgroup stores gen to *gate when it starts
 bool jit_launch_sim_gated(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                           const int* count, void* ybuf, double* info, float* dy, int n, int* gate, int gen, int gq,

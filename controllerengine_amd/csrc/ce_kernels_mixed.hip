@@ -930,10 +930,10 @@ bool launch_mark_refine(int ns, int F, const ProblemDev* P, const StageBufs& B, 
   return false;
 }
 
-// Bounded wait on the second stream for the gated simulation's start signal: lane 0 polls *gate
-// (agent-scope loads) until it reads gen, for at most `limit` polls, so the kernel always ends.
-// Returning early only lets the following kernels start sooner; nothing depends on it for
-// correctness.
+// Bounded wait on the second stream for the simulation's signal (the gated form's start, the
+// ungated form's first finished wave): lane 0 polls *gate (agent-scope loads) until it reads gen,
+// for at most `limit` polls, so the kernel always ends.  Returning early only lets the following
+// kernels start sooner; nothing depends on it for correctness.
 __global__ __launch_bounds__(64) void k_gate_wait(const int* __restrict__ gate, int gen, int limit) {
   if (threadIdx.x != 0) return;
   for (int i = 0; i < limit; i++) {
@@ -942,8 +942,8 @@ __global__ __launch_bounds__(64) void k_gate_wait(const int* __restrict__ gate, 
   }
 }
 
-void launch_gate_wait(const int* gate, int gen, hipStream_t st) {
-  hipLaunchKernelGGL(k_gate_wait, dim3(1), dim3(64), 0, st, gate, gen, 512);
+void launch_gate_wait(const int* gate, int gen, int limit, hipStream_t st) {
+  hipLaunchKernelGGL(k_gate_wait, dim3(1), dim3(64), 0, st, gate, gen, limit);
 }
 
 // crossing scan and total score (after the fit and the refinement)
