@@ -55,11 +55,26 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
   // LDL^T with symmetric pivoting on the largest remaining diagonal (lower storage)
   bool all_zero = false;
   for (int k = 0; k < N2; k++) {
+    // pivot: the first i > k whose |diag| exceeds every earlier candidate and |G2[k][k]|, else k
+    // (the sequential scan with a strict comparison: NaN never wins, and a NaN at k keeps k).
+    // Each lane takes the first maximum of its own rows, then the group keeps the largest value
+    // and, among equal values, the smallest row.
     int idx = k;
-    double best = cedm::fabs_(G2[tri(k, k)]);
-    for (int i = k + 1; i < N2; i++) {
-      const double a = cedm::fabs_(G2[tri(i, i)]);
-      if (a > best) { best = a; idx = i; }
+    {
+      double lbest = -1.0;
+      int lidx = N2;
+      for (int i = k + 1 + l; i < N2; i += G) {
+        const double a = cedm::fabs_(G2[tri(i, i)]);
+        if (a > lbest) { lbest = a; lidx = i; }
+      }
+#pragma unroll
+      for (int d = 1; d < G; d <<= 1) {
+        const double ob = __shfl_xor(lbest, d);
+        const int oi = __shfl_xor(lidx, d);
+        if (ob > lbest || (ob == lbest && oi < lidx)) { lbest = ob; lidx = oi; }
+      }
+      const double bk = cedm::fabs_(G2[tri(k, k)]);
+      if (lidx < N2 && lbest > bk) idx = lidx;
     }
     fs_sync();
     if (l == 0) trans[k] = idx;
