@@ -14,11 +14,12 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 
 // Group-parallel form: G lanes of one candidate (l = lane in group) work on LDS-resident data.
 // Element-wise steps are split over lanes by their independent index; every element sees the
-// same operations in the same order as in a sequential solve (the pivot search and the
-// triangular substitutions run on every lane / on lane 0 in the original order), so the result
-// does not depend on G.  The lanes of the group must be convergent; phases are separated by a
-// wavefront-scope fence (the group's lanes are lanes of one wave: no wait is needed, only the
-// compiler's program order).
+// same operations in the same order as in a sequential solve (the pivot search keeps the
+// sequential scan's choice, and the triangular substitutions run whole on every lane, on a
+// register copy of the vector), so the result does not depend on G.  The lanes of the group must
+// be convergent; phases are separated by a wavefront-scope fence (the group's lanes are lanes of
+// one wave: no wait is needed, only the compiler's program order).  NCP: the padded order (a
+// compile-time bound on n2).
 // G2: packed lower triangle of the 2nt x 2nt Gram (overwritten), atb: right-hand side; dg, sd, cs,
 // x, tmp: LDS vectors of 2nt; trans: LDS ints of 2nt.  u: unique modes of candidate c.
 // n2: order of the system solved, 2nt, or 2u when the rows and columns of the modes u..nt-1 are
@@ -27,7 +28,7 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 // same values).
 __device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
-template <int G>
+template <int G, int NCP>
 __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
                                                     double* row, double* G2, const double* atb, double* dg,
                                                     double* sd, double* cs, double* x, double* tmp, int* trans,
@@ -108,28 +109,77 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
     fs_sync();
   }
   fs_sync();
-  if (l == 0) {
-    if (all_zero) {
-      for (int i = 0; i < N2; i++) x[i] = 0.0;
-    } else {
-      for (int k = 0; k < N2; k++) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
-      for (int i = 0; i < N2; i++) {
-        double acc = 0.0;
-        for (int j = 0; j < i; j++) acc = cedm::fma_(G2[tri(i, j)], x[j], acc);
-        x[i] = x[i] - acc;
+  if (all_zero) {
+    for (int i = l; i < N2; i += G) x[i] = 0.0 / cs[i];
+  } else {
+    // Substitutions: P^T L D L^T P x = b, in the sequential solve's order of operations.  The
+    // swaps k = 0..N2-1 of the right-hand side leave at position i the entry that tracing i back
+    // through them reaches; each lane traces its own positions.
+    for (int i = l; i < N2; i += G) {
+      int p = i;
+      for (int k = N2 - 1; k >= 0; k--) {
+        const int t = trans[k];
+        p = (p == k) ? t : ((p == t) ? k : p);
       }
-      for (int i = 0; i < N2; i++) {
-        const double d = G2[tri(i, i)];
-        x[i] = (cedm::fabs_(d) > DBL_MIN) ? x[i] / d : 0.0;
-      }
-      for (int i = N2 - 1; i >= 0; i--) {
-        double acc = 0.0;
-        for (int j = i + 1; j < N2; j++) acc = cedm::fma_(G2[tri(j, i)], x[j], acc);
-        x[i] = x[i] - acc;
-      }
-      for (int k = N2 - 1; k >= 0; k--) { const double t = x[k]; x[k] = x[trans[k]]; x[trans[k]] = t; }
+      tmp[i] = x[p];
     }
-    for (int i = 0; i < N2; i++) x[i] = x[i] / cs[i];
+    fs_sync();
+    // Every lane of the group runs both triangular solves whole, on a register copy of the vector
+    // (loops unrolled to the padded order NCP; rows from N2 on are masked off).  Each row is the
+    // sequential dot product, fma over j in increasing order from 0.0, then one subtraction; the
+    // lanes only remove the round trips through LDS.  Entries from N2 on hold -0.0 and the Gram
+    // rows from N2 on hold +0.0, so a padded term fma(+0, -0, acc) = acc exactly, signed zeros
+    // and NaN included.
+    double xr[NCP];
+#pragma unroll
+    for (int i = 0; i < NCP; i++) xr[i] = (i < N2) ? tmp[i] : -0.0;
+    // L y = P b (unit lower; row 0 is y0 = b0 - 0.0 = b0)
+#pragma unroll
+    for (int i = 1; i < NCP; i++) {
+      if (i < N2) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < i; j++) acc = cedm::fma_(G2[tri(i, j)], xr[j], acc);
+        xr[i] = xr[i] - acc;
+      }
+    }
+    fs_sync();
+    // D z = y: one division per entry, on the lane that owns the entry
+#pragma unroll
+    for (int i = 0; i < NCP; i++)
+      if (i % G == l && i < N2) tmp[i] = xr[i];
+    fs_sync();
+    for (int i = l; i < N2; i += G) {
+      const double d = G2[tri(i, i)];
+      tmp[i] = (cedm::fabs_(d) > DBL_MIN) ? tmp[i] / d : 0.0;
+    }
+    fs_sync();
+#pragma unroll
+    for (int i = 0; i < NCP; i++) xr[i] = (i < N2) ? tmp[i] : -0.0;
+    // L^T w = z (row N2 - 1 is w = z - 0.0 = z)
+#pragma unroll
+    for (int i = NCP - 2; i >= 0; i--) {
+      if (i + 1 < N2) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < NCP; j++) acc = cedm::fma_(G2[tri(j, i)], xr[j], acc);
+        xr[i] = xr[i] - acc;
+      }
+    }
+    fs_sync();
+#pragma unroll
+    for (int i = 0; i < NCP; i++)
+      if (i % G == l && i < N2) tmp[i] = xr[i];
+    fs_sync();
+    // x = P^T w (the swaps k = N2-1..0), then the column scaling
+    for (int i = l; i < N2; i += G) {
+      int p = i;
+      for (int k = 0; k < N2; k++) {
+        const int t = trans[k];
+        p = (p == k) ? t : ((p == t) ? k : p);
+      }
+      x[i] = tmp[p] / cs[i];
+    }
   }
   fs_sync();
 

@@ -400,6 +400,9 @@ __device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P
   // the columns of modes u..NM-1 stay zero; each round evaluates only the u x TB basis values the
   // candidate has, spread over the group's lanes (same values, fewer idle evaluations)
   for (int k = g; k < TB * NCP; k += G) phi[slot][k / NCP][k % NCP] = 0.0;
+  // Gram rows beyond 2nt (padding, never accumulated) are +0.0: the solve's unrolled substitution
+  // reads them as exact no-op terms
+  for (int k = nc2 * (nc2 + 1) / 2 + g; k < S::NTRI; k += G) gram[slot][k] = 0.0;
   const int ue = active ? u : 0;
   __syncthreads();
   for (int tb = 0; tb < cnt; tb += TB) {
@@ -493,7 +496,7 @@ __device__ __forceinline__ void k_fit_fp64_body(const ProblemDev* __restrict__ P
   }
   double* sc = row + P->info_scoring;
   if (sc[0] != 0.0) return;  // modalsolve skips flagged candidates (src/modalsolve.cu:112)
-  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g, 2 * nt);
+  solve_and_score_grp<G, NCP>(P, B, c, u, row, G2, atb, dg, sd, cs, x, vec[slot][5], itr[slot], g, 2 * nt);
 }
 
 template <int NB>

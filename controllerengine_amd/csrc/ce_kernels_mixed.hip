@@ -294,7 +294,7 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
     for (int i = 0; i < nc2; i++) db[i] = (float)atb[i];
   }
   if (row[P->info_scoring] != 0.0) return;
-  solve_and_score_grp<G>(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4],
+  solve_and_score_grp<G, NCP>(P, B, c, u, row, G2, atb, vec[slot][1], vec[slot][2], vec[slot][3], vec[slot][4],
                          vec[slot][5], itr[slot], g, 2 * u);
 }
 
