@@ -28,11 +28,44 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 // same values).
 __device__ __forceinline__ void fs_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// Group reduction to the largest v, the smallest i among equal v (v never NaN): every lane of
+// the group ends with the result.  The selection is a total order, so any pairing gives the same
+// winner.  For 16-lane groups (the rows of the wave) the partner comes by DPP: lane ^ 1 and
+// lane ^ 2 (quad permutes), then the mirrored lane within 8 and within 16 (row_half_mirror,
+// row_mirror), each step joining two disjoint halves; other sizes pair lanes through the LDS
+// crossbar.
+template <int CTRL>
+__device__ __forceinline__ void argmax_first_dpp(double& v, int& i) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+  const double ov = __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  const int oi = __builtin_amdgcn_mov_dpp(i, CTRL, 0xF, 0xF, true);
+  if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+template <int G>
+__device__ __forceinline__ void grp_argmax_first(double& v, int& i) {
+  if constexpr (G == 16) {
+    argmax_first_dpp<0xB1>(v, i);   // quad_perm [1,0,3,2]
+    argmax_first_dpp<0x4E>(v, i);   // quad_perm [2,3,0,1]
+    argmax_first_dpp<0x141>(v, i);  // row_half_mirror
+    argmax_first_dpp<0x140>(v, i);  // row_mirror
+  } else {
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      const double ov = __shfl_xor(v, d);
+      const int oi = __shfl_xor(i, d);
+      if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+  }
+}
+
 template <int G, int NCP>
 __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
                                                     double* row, double* G2, const double* atb, double* dg,
                                                     double* sd, double* cs, double* x, double* tmp, int* trans,
                                                     int l, int n2) {
+  static_assert(NCP <= 2 * G, "at most two rows of the system per lane");
   const int nt = P->nt, N2 = n2;
   const int n = B.n;
   double* sc = row + P->info_scoring;
@@ -68,12 +101,7 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
         const double a = cedm::fabs_(G2[tri(i, i)]);
         if (a > lbest) { lbest = a; lidx = i; }
       }
-#pragma unroll
-      for (int d = 1; d < G; d <<= 1) {
-        const double ob = __shfl_xor(lbest, d);
-        const int oi = __shfl_xor(lidx, d);
-        if (ob > lbest || (ob == lbest && oi < lidx)) { lbest = ob; lidx = oi; }
-      }
+      grp_argmax_first<G>(lbest, lidx);
       const double bk = cedm::fabs_(G2[tri(k, k)]);
       if (lidx < N2 && lbest > bk) idx = lidx;
     }
@@ -89,23 +117,60 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
     if (k > 0) {
       for (int j = l; j < k; j += G) tmp[j] = G2[tri(j, j)] * G2[tri(k, j)];
       fs_sync();
-      if (l == 0) {
-        double acc = 0.0;
-        for (int j = 0; j < k; j++) acc = cedm::fma_(G2[tri(k, j)], tmp[j], acc);
-        G2[tri(k, k)] = G2[tri(k, k)] - acc;
+      // column k: rows k + l and k + l + G (row k is the diagonal), each the sequential dot
+      // product over j < k from 0.0 then one subtraction; the two rows' chains are interleaved
+      // and their operands loaded four terms at a time (a lane without a second row reads row k
+      // and drops the result)
+      const int r0 = k + l, r1 = r0 + G;
+      const double* p0 = G2 + tri(r0 < N2 ? r0 : k, 0);
+      const double* p1 = G2 + tri(r1 < N2 ? r1 : k, 0);
+      double a0 = 0.0, a1 = 0.0;
+      int j = 0;
+      for (; j + 4 <= k; j += 4) {
+        const double t0 = tmp[j], t1 = tmp[j + 1], t2 = tmp[j + 2], t3 = tmp[j + 3];
+        const double x0 = p0[j], x1 = p0[j + 1], x2 = p0[j + 2], x3 = p0[j + 3];
+        const double y0 = p1[j], y1 = p1[j + 1], y2 = p1[j + 2], y3 = p1[j + 3];
+        a0 = cedm::fma_(x0, t0, a0);
+        a1 = cedm::fma_(y0, t0, a1);
+        a0 = cedm::fma_(x1, t1, a0);
+        a1 = cedm::fma_(y1, t1, a1);
+        a0 = cedm::fma_(x2, t2, a0);
+        a1 = cedm::fma_(y2, t2, a1);
+        a0 = cedm::fma_(x3, t3, a0);
+        a1 = cedm::fma_(y3, t3, a1);
       }
-      for (int i = k + 1 + l; i < N2; i += G) {
-        double a = 0.0;
-        for (int j = 0; j < k; j++) a = cedm::fma_(G2[tri(i, j)], tmp[j], a);
-        G2[tri(i, k)] = G2[tri(i, k)] - a;
+      if (j < k) {  // the last one to three terms, loaded together
+        const int m = k - j;
+        const int j1 = (m > 1) ? j + 1 : j, j2 = (m > 2) ? j + 2 : j;
+        const double t0 = tmp[j], t1 = tmp[j1], t2 = tmp[j2];
+        const double x0 = p0[j], x1 = p0[j1], x2 = p0[j2];
+        const double y0 = p1[j], y1 = p1[j1], y2 = p1[j2];
+        a0 = cedm::fma_(x0, t0, a0);
+        a1 = cedm::fma_(y0, t0, a1);
+        if (m > 1) {
+          a0 = cedm::fma_(x1, t1, a0);
+          a1 = cedm::fma_(y1, t1, a1);
+        }
+        if (m > 2) {
+          a0 = cedm::fma_(x2, t2, a0);
+          a1 = cedm::fma_(y2, t2, a1);
+        }
       }
+      if (r0 < N2) G2[tri(r0, k)] = G2[tri(r0, k)] - a0;
+      if (r1 < N2) G2[tri(r1, k)] = G2[tri(r1, k)] - a1;
       fs_sync();
     }
     const double akk = G2[tri(k, k)];
     const bool valid = cedm::fabs_(akk) > 0.0;
     if (k == 0 && !valid) { all_zero = true; break; }
-    if (valid)
-      for (int i = k + 1 + l; i < N2; i += G) G2[tri(i, k)] = G2[tri(i, k)] / akk;
+    if (valid) {
+      // rows k + 1 + l and k + 1 + l + G of column k, both loads ahead of the divisions
+      const int i0 = k + 1 + l, i1 = i0 + G;
+      const double v0 = (i0 < N2) ? G2[tri(i0, k)] : 0.0;
+      const double v1 = (i1 < N2) ? G2[tri(i1, k)] : 0.0;
+      if (i0 < N2) G2[tri(i0, k)] = v0 / akk;
+      if (i1 < N2) G2[tri(i1, k)] = v1 / akk;
+    }
     fs_sync();
   }
   fs_sync();
