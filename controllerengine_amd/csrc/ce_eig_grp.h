@@ -46,12 +46,27 @@ __device__ __forceinline__ void grp_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ
 
 // Maxima over the G lanes of a group (lanes gi G .. gi G + G - 1 of the wave, which run the same
 // control flow): exact and independent of the combination order.
+// Groups of 4, 8 and 16 lanes are quads, half rows and rows of the wave: the partners come by
+// DPP (lane ^ 1 and lane ^ 2 by quad permutes, then the mirrored lane within 8 and within 16),
+// each step joining two disjoint halves; other sizes pair lanes through the LDS crossbar.
+template <int CTRL>
+__device__ __forceinline__ int max_i_dpp(int v) {
+  const int o = __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+  return v > o ? v : o;
+}
 template <int G>
 __device__ __forceinline__ int grp_max_i(int v) {
+  if constexpr (G == 4 || G == 8 || G == 16) {
+    v = max_i_dpp<0xB1>(v);                       // quad_perm [1,0,3,2]
+    v = max_i_dpp<0x4E>(v);                       // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) v = max_i_dpp<0x141>(v);  // row_half_mirror
+    if constexpr (G == 16) v = max_i_dpp<0x140>(v); // row_mirror
+  } else {
 #pragma unroll
-  for (int d = 1; d < G; d <<= 1) {
-    const int o = __shfl_xor(v, d);
-    v = v > o ? v : o;
+    for (int d = 1; d < G; d <<= 1) {
+      const int o = __shfl_xor(v, d);
+      v = v > o ? v : o;
+    }
   }
   return v;
 }
