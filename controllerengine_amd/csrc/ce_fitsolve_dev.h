@@ -60,6 +60,20 @@ __device__ __forceinline__ void grp_argmax_first(double& v, int& i) {
   }
 }
 
+// The value v holds in lane 0 of the caller's group, in every lane of the group (DPP row
+// broadcast for 16-lane groups, the LDS crossbar otherwise).
+template <int G>
+__device__ __forceinline__ double grp_lane0(double v) {
+  if constexpr (G == 16) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150, 0xF, 0xF, true);  // row_newbcast:0
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  } else {
+    return __shfl(v, (int)(threadIdx.x & ~(unsigned)(G - 1)));
+  }
+}
+
 template <int G, int NCP>
 __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict__ P, const StageBufs& B, int c, int u,
                                                     double* row, double* G2, const double* atb, double* dg,
@@ -89,41 +103,67 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
   // LDL^T with symmetric pivoting on the largest remaining diagonal (lower storage)
   bool all_zero = false;
   for (int k = 0; k < N2; k++) {
+    // Lane l owns rows k + l and k + l + G of the step (NCP <= 2G: every row from k on).
+    const int r0 = k + l, r1 = r0 + G;
+    const bool h0 = r0 < N2, h1 = r1 < N2;
     // pivot: the first i > k whose |diag| exceeds every earlier candidate and |G2[k][k]|, else k
     // (the sequential scan with a strict comparison: NaN never wins, and a NaN at k keeps k).
-    // Each lane takes the first maximum of its own rows, then the group keeps the largest value
-    // and, among equal values, the smallest row.
+    // Each lane takes the first maximum of its own rows k + 1 + l, k + 1 + l + G, then the group
+    // keeps the largest value and, among equal values, the smallest row.  The diagonal entries are
+    // loaded together (addresses clamped into the matrix, values past its end dropped).
     int idx = k;
     {
+      const int i0 = r0 + 1, i1 = r1 + 1;
+      const double bk = cedm::fabs_(G2[tri(k, k)]);
+      const double e0 = cedm::fabs_(G2[tri(i0 < N2 ? i0 : k, i0 < N2 ? i0 : k)]);
+      const double e1 = cedm::fabs_(G2[tri(i1 < N2 ? i1 : k, i1 < N2 ? i1 : k)]);
       double lbest = -1.0;
       int lidx = N2;
-      for (int i = k + 1 + l; i < N2; i += G) {
-        const double a = cedm::fabs_(G2[tri(i, i)]);
-        if (a > lbest) { lbest = a; lidx = i; }
-      }
+      if (i0 < N2 && e0 > lbest) { lbest = e0; lidx = i0; }
+      if (i1 < N2 && e1 > lbest) { lbest = e1; lidx = i1; }
       grp_argmax_first<G>(lbest, lidx);
-      const double bk = cedm::fabs_(G2[tri(k, k)]);
       if (lidx < N2 && lbest > bk) idx = lidx;
     }
     fs_sync();
     if (l == 0) trans[k] = idx;
     if (k != idx) {
-      for (int j = l; j < k; j += G) { const double t = G2[tri(k, j)]; G2[tri(k, j)] = G2[tri(idx, j)]; G2[tri(idx, j)] = t; }
-      for (int i = idx + 1 + l; i < N2; i += G) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(i, idx)]; G2[tri(i, idx)] = t; }
-      if (l == 0) { const double t = G2[tri(k, k)]; G2[tri(k, k)] = G2[tri(idx, idx)]; G2[tri(idx, idx)] = t; }
-      for (int i = k + 1 + l; i < idx; i += G) { const double t = G2[tri(i, k)]; G2[tri(i, k)] = G2[tri(idx, i)]; G2[tri(idx, i)] = t; }
+      // symmetric swap of rows / columns k and idx.  The four parts touch disjoint entries (row k
+      // and row idx left of k; column k and column idx below idx; the two diagonal entries; column k
+      // and row idx between k and idx), each entry on one lane: every load is issued before the
+      // first store.
+      const int ja = l, jb = l + G;
+      const int pa = idx + 1 + l, pb = pa + G;
+      const int qa = k + 1 + l, qb = qa + G;
+      const bool hja = ja < k, hjb = jb < k, hpa = pa < N2, hpb = pb < N2, hqa = qa < idx, hqb = qb < idx;
+      double s1a = 0.0, s2a = 0.0, s1b = 0.0, s2b = 0.0, u1a = 0.0, u2a = 0.0, u1b = 0.0, u2b = 0.0;
+      double w1a = 0.0, w2a = 0.0, w1b = 0.0, w2b = 0.0, dk = 0.0, di = 0.0;
+      if (hja) { s1a = G2[tri(k, ja)]; s2a = G2[tri(idx, ja)]; }
+      if (hjb) { s1b = G2[tri(k, jb)]; s2b = G2[tri(idx, jb)]; }
+      if (hpa) { u1a = G2[tri(pa, k)]; u2a = G2[tri(pa, idx)]; }
+      if (hpb) { u1b = G2[tri(pb, k)]; u2b = G2[tri(pb, idx)]; }
+      if (hqa) { w1a = G2[tri(qa, k)]; w2a = G2[tri(idx, qa)]; }
+      if (hqb) { w1b = G2[tri(qb, k)]; w2b = G2[tri(idx, qb)]; }
+      if (l == 0) { dk = G2[tri(k, k)]; di = G2[tri(idx, idx)]; }
+      if (hja) { G2[tri(k, ja)] = s2a; G2[tri(idx, ja)] = s1a; }
+      if (hjb) { G2[tri(k, jb)] = s2b; G2[tri(idx, jb)] = s1b; }
+      if (hpa) { G2[tri(pa, k)] = u2a; G2[tri(pa, idx)] = u1a; }
+      if (hpb) { G2[tri(pb, k)] = u2b; G2[tri(pb, idx)] = u1b; }
+      if (hqa) { G2[tri(qa, k)] = w2a; G2[tri(idx, qa)] = w1a; }
+      if (hqb) { G2[tri(qb, k)] = w2b; G2[tri(idx, qb)] = w1b; }
+      if (l == 0) { G2[tri(k, k)] = di; G2[tri(idx, idx)] = dk; }
       fs_sync();
     }
+    // column k: rows r0 and r1 (row k, the diagonal, on lane 0), each the sequential dot product
+    // over j < k from 0.0 then one subtraction; the two rows' chains are interleaved and their
+    // operands loaded four terms at a time (a lane without a second row reads row k and drops the
+    // result).  The new column stays in registers: the pivot comes from lane 0 by DPP, and the
+    // division below the diagonal follows before the one store.
+    double c0 = G2[tri(h0 ? r0 : k, k)], c1 = G2[tri(h1 ? r1 : k, k)];
     if (k > 0) {
       for (int j = l; j < k; j += G) tmp[j] = G2[tri(j, j)] * G2[tri(k, j)];
       fs_sync();
-      // column k: rows k + l and k + l + G (row k is the diagonal), each the sequential dot
-      // product over j < k from 0.0 then one subtraction; the two rows' chains are interleaved
-      // and their operands loaded four terms at a time (a lane without a second row reads row k
-      // and drops the result)
-      const int r0 = k + l, r1 = r0 + G;
-      const double* p0 = G2 + tri(r0 < N2 ? r0 : k, 0);
-      const double* p1 = G2 + tri(r1 < N2 ? r1 : k, 0);
+      const double* p0 = G2 + tri(h0 ? r0 : k, 0);
+      const double* p1 = G2 + tri(h1 ? r1 : k, 0);
       double a0 = 0.0, a1 = 0.0;
       int j = 0;
       for (; j + 4 <= k; j += 4) {
@@ -156,21 +196,14 @@ __device__ __forceinline__ void solve_and_score_grp(const ProblemDev* __restrict
           a1 = cedm::fma_(y2, t2, a1);
         }
       }
-      if (r0 < N2) G2[tri(r0, k)] = G2[tri(r0, k)] - a0;
-      if (r1 < N2) G2[tri(r1, k)] = G2[tri(r1, k)] - a1;
-      fs_sync();
+      c0 = c0 - a0;
+      c1 = c1 - a1;
     }
-    const double akk = G2[tri(k, k)];
+    const double akk = grp_lane0<G>(c0);
     const bool valid = cedm::fabs_(akk) > 0.0;
     if (k == 0 && !valid) { all_zero = true; break; }
-    if (valid) {
-      // rows k + 1 + l and k + 1 + l + G of column k, both loads ahead of the divisions
-      const int i0 = k + 1 + l, i1 = i0 + G;
-      const double v0 = (i0 < N2) ? G2[tri(i0, k)] : 0.0;
-      const double v1 = (i1 < N2) ? G2[tri(i1, k)] : 0.0;
-      if (i0 < N2) G2[tri(i0, k)] = v0 / akk;
-      if (i1 < N2) G2[tri(i1, k)] = v1 / akk;
-    }
+    if (h0) G2[tri(r0, k)] = (valid && l > 0) ? c0 / akk : c0;
+    if (h1) G2[tri(r1, k)] = valid ? c1 / akk : c1;
     fs_sync();
   }
   fs_sync();
