@@ -101,6 +101,9 @@ struct ce_engine {
   // simulation and the fit (forked after the compaction, joined before the crossing scan)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // split simulation (sim_split_slots): the remainder and its fit on a third stream
+  hipStream_t rem = nullptr;
+  hipEvent_t ev_sima = nullptr, ev_rem = nullptr;
   // start signal of the gated simulation (one int, value = gate_gen of the last gated launch)
   Buf gate;
   int gate_gen = 0;
@@ -350,6 +353,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   E->last_sim_jit = 0;
   const int hslot = E->hist_a.empty() ? -1 : E->hist_pos;
   if (hslot >= 0) HIPCHK(hipEventRecord(E->hist_a[hslot], st));
+  bool split = false;
   if (fn) {
     const bool st_form = E->sim_fn_struct[mixed ? 1 : 0][gated ? 1 : 0];
     bool ok;
@@ -360,11 +364,29 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     int gq = 1;
     if (mixed && !st_form && lanes[0] && lanes[1])
       gq = (4 * sim_grid <= 2 * E->n_simd) ? 4 : ((2 * sim_grid <= 2 * E->n_simd) ? 2 : 1);
+    // split simulation: ungated one-lane form with room for at least two full rounds (the device
+    // decides from the live count whether a small enough remainder is left, sim_split_slots)
+    split = fork && !gated && !st_form && mixed && gq == 1 && lanes[1] && sim_grid >= 2 * E->n_simd;
+    if (split && !E->rem) {
+      HIPCHK(hipStreamCreateWithFlags(&E->rem, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&E->ev_sima, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&E->ev_rem, hipEventDisableTiming));
+    }
     if (st_form) {
       ok = gated ? jit_launch_sim_struct_gated(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y,
                                                n, (int*)E->gate.p, E->gate_gen, P)
                  : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P,
                                          fork ? (int*)E->gate.p : nullptr, E->gate_gen);
+    } else if (split) {
+      // the one-lane kernel's full rounds here; the remainder (4 lanes per candidate) and its fit on
+      // the third stream once this kernel has finished, so that no SIMD holds a wave more than the
+      // others beside it
+      ok = jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, 1, E->n_simd,
+                          (int*)E->gate.p, E->gate_gen, 1);
+      HIPCHK(hipEventRecord(E->ev_sima, st));
+      HIPCHK(hipStreamWaitEvent(E->rem, E->ev_sima, 0));
+      ok = ok && jit_launch_sim(lanes[1], 2 * E->n_simd, E->rem, B.gains, B.flags, B.list, B.count, ybuf, B.info,
+                                B.d_y, n, 4, E->n_simd, nullptr, 0, 2);
     } else {
       ok = true;
       for (int G : {4, 2, 1}) {
@@ -387,14 +409,26 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     if (!ok) throw std::runtime_error("no simulation kernel for these dimensions");
   }
   if (hslot >= 0) {
-    HIPCHK(hipEventRecord(E->hist_b[hslot], st));
+    // the stage ends with the remainder when the simulation is split
+    HIPCHK(hipEventRecord(E->hist_b[hslot], split ? E->rem : st));
     E->hist_pos = (hslot + 1) % (int)E->hist_a.size();
     E->hist_n = std::min(E->hist_n + 1, (int)E->hist_a.size());
   }
   HIPCHK(hipEventRecord(E->ev[2], st));
-  // stage 3: modal fit and behaviour score
-  if (!(mixed ? launch_fit_mixed(nt, P, B, st) : launch_fit_fp64(nt, P, B, st)))
+  // stage 3: modal fit and behaviour score (split simulation: each part's candidates on the stream
+  // that simulated them, joined before the row updates)
+  if (split) {
+    StageBufs B1 = B, B2 = B;
+    B1.sim_part = 1;
+    B2.sim_part = 2;
+    B1.sim_cap = B2.sim_cap = E->n_simd;
+    if (!launch_fit_mixed(nt, P, B2, E->rem)) throw std::runtime_error("no fit kernel for these dimensions");
+    HIPCHK(hipEventRecord(E->ev_rem, E->rem));
+    if (!launch_fit_mixed(nt, P, B1, st)) throw std::runtime_error("no fit kernel for these dimensions");
+    HIPCHK(hipStreamWaitEvent(st, E->ev_rem, 0));
+  } else if (!(mixed ? launch_fit_mixed(nt, P, B, st) : launch_fit_fp64(nt, P, B, st))) {
     throw std::runtime_error("no fit kernel for these dimensions");
+  }
   HIPCHK(hipEventRecord(E->ev[3], st));
   // stage 4: margin sweep, crossings and total score
   if (fork) {
@@ -449,6 +483,9 @@ extern "C" CE_EXPORT void ce_engine_destroy(ce_engine* E) {
   if (E->ev_fork) hipEventDestroy(E->ev_fork);
   if (E->ev_join) hipEventDestroy(E->ev_join);
   if (E->side) hipStreamDestroy(E->side);
+  if (E->ev_sima) hipEventDestroy(E->ev_sima);
+  if (E->ev_rem) hipEventDestroy(E->ev_rem);
+  if (E->rem) hipStreamDestroy(E->rem);
   if (E->events_ok)
     for (int i = 0; i < N_EVENTS; i++) hipEventDestroy(E->ev[i]);
   delete E;

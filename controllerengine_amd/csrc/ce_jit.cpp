@@ -548,6 +548,12 @@ static const char* kSimArgs =
     "struct SimArgs { const float* gains; const int* flags; const int* list; const int* count; void* ybuf;\n"
     "  double* info; float* dy; int n; int pad; };\n"
     "template <int B> struct BC { __device__ constexpr operator bool() const { return B != 0; } };\n"
+    // first list slot of the split simulation's remainder (the rule of sim_split_slots, ce_stage.h)
+    "static __device__ inline int ce_split(int live, int cap) {\n"
+    "  if (cap <= 0) return live;\n"
+    "  const int w = (live + 63) / 64;\n"
+    "  const int k = w / cap, r = w - k * cap;\n"
+    "  return (k >= 2 && r > 0 && 2 * r <= cap) ? k * cap * 64 : live; }\n"
     // fmin/fmax as the bare instructions: every operand here is an arithmetic result, a constant or
     // an infinity, never a signalling NaN, so they return what fmin/fmax return, without the
     // canonicalising moves the compiler adds for operands it cannot prove quiet
@@ -611,6 +617,8 @@ static void gen_struct_body(std::ostringstream& o, const ProblemDev& P, bool mix
 static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool structural) {
   // lane groups: the float literal kernel also carries 2- and 4-lane forms of the step
   const bool grp = mixed && !structural;
+  // the ungated literal kernels take the part of a split simulation (0: every slot)
+  const bool split = !gated && !structural;
   Gen g;
   g.f32 = mixed;
   std::ostringstream& o = g.o;
@@ -644,20 +652,35 @@ static std::string gen_single(const ProblemDev& P, bool mixed, bool gated, bool 
          "    const int* __restrict__ list, const int* __restrict__ count, void* __restrict__ ybuf, double* __restrict__ info,\n"
          "    float* __restrict__ dy, int n" << (gated ? ", int* __restrict__ gate, int gate_gen" : "")
       << (structural ? ", const char* __restrict__ probp" : ", int gq, int cap")
-      << (gated ? "" : ", int* __restrict__ done, int done_gen") << ") {\n"
+      << (gated ? "" : ", int* __restrict__ done, int done_gen") << (split ? ", int part" : "") << ") {\n"
          "  const SimArgs A{gains, flags, list, count, ybuf, info, dy, n, 0};\n";
+    // part 2 (split): the remainder, from the split slot on, in this kernel's lanes; part 1: the
+    // one-lane kernel's full rounds only
+    if (split)
+      o << "  int bxo = blockIdx.x;\n"
+           "  if (part == 2) {\n"
+           "    const int live2 = __builtin_amdgcn_readfirstlane(*count);\n"
+           "    const int sp = ce_split(live2, cap);\n"
+           "    if (sp >= live2) return;\n"  // no remainder: every slot is the one-lane kernel's
+           "    bxo += sp / " << 64 / G << ";\n"
+           "  } else {\n";
     if (grp)
       o << "  const int live = __builtin_amdgcn_readfirstlane(*count);\n"
            "  const int pick = (gq >= 4 && live <= 16 * cap) ? 4 : ((gq >= 2 && live <= 32 * cap) ? 2 : 1);\n"
            "  if (pick != " << G << ") return;\n";
+    if (split)
+      o << "  if (part == 1 && (int)blockIdx.x * " << 64 / G << " >= ce_split(__builtin_amdgcn_readfirstlane(*count), cap))"
+           " return;\n"
+           "  }\n";
     if (gated)
       o << "  asm volatile(\"\" ::: \"v255\", \"a255\");\n"
            "  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)\n"
            "    __hip_atomic_store(gate, gate_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
     o << "  __builtin_amdgcn_s_setprio(3);\n";
     if (!gated) o << "  const int cnt1 = __builtin_amdgcn_readfirstlane(*count);\n";
-    if (G == 1) o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", blockIdx.x);\n";
-    else o << "  sim_g" << G << "(A, blockIdx.x);\n";
+    const char* bxs = split ? "bxo" : "blockIdx.x";
+    if (G == 1) o << "  sim_one(A, " << (structural ? "probp" : "nullptr") << ", " << bxs << ");\n";
+    else o << "  sim_g" << G << "(A, " << bxs << ");\n";
     // ungated: the first working wave to finish (or block 0 when none is live) signals done_gen
     if (!gated)
       o << "  if (done != nullptr && threadIdx.x == 0 && ((int)blockIdx.x * " << 64 / G
@@ -891,8 +914,8 @@ bool jit_get(JitCache* cache, const std::string& src, void** fn_out, std::string
 
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap, int* done,
-                    int done_gen) {
-  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gq, &cap, &done, &done_gen};
+                    int done_gen, int part) {
+  void* args[] = {&gains, &flags, &list, &count, &ybuf, &info, &dy, &n, &gq, &cap, &done, &done_gen, &part};
   return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, 64, 1, 1, 0, st, args, nullptr) == hipSuccess;
 }
 

@@ -121,9 +121,17 @@ __device__ __forceinline__ void atb_run(const ProblemDev* __restrict__ P, const 
     if (e0 + k < u) out[(size_t)k * n] = make_double2(ar[k], ai[k]);
 }
 
+// list slot idx belongs to the part of the list this launch covers (split simulation)
+__device__ __forceinline__ bool in_sim_part(const StageBufs& B, int idx, int live) {
+  if (B.sim_part == 0) return true;
+  const int lim = sim_split_slots(live, B.sim_cap);
+  return (B.sim_part == 1) ? idx < lim : idx >= lim;
+}
+
 __device__ __forceinline__ void k_atb_body(const ProblemDev* __restrict__ P, const StageBufs& B, int bx, int nbx, int by, int bz) {
   const int idx = bx * 64 + threadIdx.x;
-  if (idx >= *B.count) return;
+  const int live = *B.count;
+  if (idx >= live || !in_sim_part(B, idx, live)) return;
   const int c = B.list[idx];
   const double* row = B.info + (size_t)c * P->info_w;
   if (B.flags[c] & F_UNSTABLE) return;
@@ -184,7 +192,8 @@ __device__ __forceinline__ void k_fitcf_body(const ProblemDev* __restrict__ P, c
   bool active = false;
   int u = 0, c = 0;
   double* row = nullptr;
-  if (idx < *B.count) {
+  const int live = *B.count;
+  if (idx < live && in_sim_part(B, idx, live)) {
     c = B.list[idx];
     row = B.info + (size_t)c * P->info_w;
     const bool stable = (B.flags[c] & F_UNSTABLE) == 0;

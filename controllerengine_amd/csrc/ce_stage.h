@@ -57,6 +57,9 @@ struct StageBufs {
   // layout [part][mode][list slot]
   double2* atbp;
   int32_t atb_parts, atb_chunk;
+  // list slots a fit launch covers (mixed chain, split simulation): 0 every slot, 1 the slots below
+  // sim_split_slots(count, sim_cap), 2 the slots from there on
+  int32_t sim_part, sim_cap;
   // diagnostics in the reference dump layout (float, candidate-major), NULL when not requested
   float* d_y;
   float* d_lina;
@@ -64,6 +67,19 @@ struct StageBufs {
   float* d_ata;
   float* d_atb;
 };
+
+// Split simulation (mixed chain, more than two rounds of one-lane waves): the one-lane kernel takes
+// the list slots of its full rounds, one 64-candidate wave per SIMD per round (cap SIMDs), and a
+// kernel with 4 lanes per candidate the rest, on another stream once the first has finished.
+// Without the split, the SIMDs that hold one wave more than the others (c3: 287 of 1,024 hold a
+// third) set the simulation's time while the rest wait.  Returns the first slot of the remainder,
+// or live (no split) with fewer than two full rounds or a remainder over half a round.
+__host__ __device__ inline int sim_split_slots(int live, int cap) {
+  if (cap <= 0) return live;
+  const int w = (live + 63) / 64;
+  const int k = w / cap, r = w - k * cap;
+  return (k >= 2 && r > 0 && 2 * r <= cap) ? k * cap * 64 : live;
+}
 
 // Lane layout of the eigen kernels.  Matrices of order <= 4, <= 8 and <= 16 (the kernel's key)
 // use 4, EIG8_G and EIG16_G lanes per candidate, with key / lanes rows on a lane.  At order 16,

@@ -100,11 +100,14 @@ def test_mixed_lane_groups_equal_interpreted(engine, name, n, lanes, monkeypatch
     assert same.all(), "lane-group and interpreted float kernels differ in %d cells" % int((~same).sum())
 
 
-@pytest.mark.parametrize("name,n", [("c2", 4096), ("c3", 1024)])
+@pytest.mark.parametrize("name,n", [("c2", 4096), ("c3", 1024), ("c3", 262144)])
 def test_mixed_two_streams_equal_one_stream(engine, name, n):
     """The default mixed chain runs the nominal eigensolve, the sweep and the refinement on a second
     stream beside a simulation that holds whole SIMDs and signals its start; with per-kernel timing on,
-    every kernel runs in order on one stream with the plain simulation kernel.  Same rows, bit for bit."""
+    every kernel runs in order on one stream with the plain simulation kernel.  Same rows, bit for bit.
+    The full c3 block (149,427 simulated candidates: two full rounds of one-lane waves on the
+    device's 1,024 SIMDs and a remainder of 287 waves) also splits the simulation: the remainder runs
+    with 4 lanes per candidate on a third stream, with its own fit."""
     p = workloads.CONFIGS[name](n=n)
     print("case", name, n, flush=True)
     _, two = engine.evaluate(p, PRECISION_MIXED)
