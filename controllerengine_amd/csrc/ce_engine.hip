@@ -366,7 +366,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
       gq = (4 * sim_grid <= 2 * E->n_simd) ? 4 : ((2 * sim_grid <= 2 * E->n_simd) ? 2 : 1);
     // split simulation: ungated one-lane form with room for at least two full rounds (the device
     // decides from the live count whether a small enough remainder is left, sim_split_slots)
-    split = fork && !gated && !st_form && mixed && gq == 1 && lanes[1] && sim_grid >= 2 * E->n_simd;
+    split = fork && !gated && !st_form && mixed && gq == 1 && lanes[0] && sim_grid >= 2 * E->n_simd;
     if (split && !E->rem) {
       HIPCHK(hipStreamCreateWithFlags(&E->rem, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&E->ev_sima, hipEventDisableTiming));
@@ -378,14 +378,15 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
                  : jit_launch_sim_struct(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, P,
                                          fork ? (int*)E->gate.p : nullptr, E->gate_gen);
     } else if (split) {
-      // the one-lane kernel's full rounds here; the remainder (4 lanes per candidate) and its fit on
-      // the third stream once this kernel has finished, so that no SIMD holds a wave more than the
-      // others beside it
+      // the one-lane kernel's full rounds here; the remainder (2 lanes per candidate: less than half a
+      // round of one-lane waves, so at most one 2-lane wave per SIMD) and its fit on the third stream
+      // once this kernel has finished, so that no SIMD holds a wave more than the others beside it
+      // (c3, same box: 4 lanes per candidate 16.19 ms per step, 2 lanes 16.00)
       ok = jit_launch_sim(fn, sim_grid, st, B.gains, B.flags, B.list, B.count, ybuf, B.info, B.d_y, n, 1, E->n_simd,
                           (int*)E->gate.p, E->gate_gen, 1);
       HIPCHK(hipEventRecord(E->ev_sima, st));
       HIPCHK(hipStreamWaitEvent(E->rem, E->ev_sima, 0));
-      ok = ok && jit_launch_sim(lanes[1], 2 * E->n_simd, E->rem, B.gains, B.flags, B.list, B.count, ybuf, B.info,
+      ok = ok && jit_launch_sim(lanes[0], E->n_simd, E->rem, B.gains, B.flags, B.list, B.count, ybuf, B.info,
                                 B.d_y, n, 4, E->n_simd, nullptr, 0, 2);
     } else {
       ok = true;

@@ -42,7 +42,7 @@ bool jit_launch_sim_struct_gated(void* fn, int grid, hipStream_t st, const float
 // candidate), cap: waves the device runs one per SIMD.  The float kernel runs the most lanes per
 // candidate, up to gq, whose live candidates' waves stay within cap; the FP64 kernel runs one.
 // part (split simulation, sim_split_slots): 0 every list slot; 1 the one-lane kernel on the slots
-// before the split; 2 the kernel launched (4 lanes) on the slots from the split on, whatever gq.
+// before the split; 2 the kernel launched (2 lanes) on the slots from the split on, whatever gq.
 bool jit_launch_sim(void* fn, int grid, hipStream_t st, const float* gains, const int* flags, const int* list,
                     const int* count, void* ybuf, double* info, float* dy, int n, int gq, int cap, int* done = nullptr,
                     int done_gen = 0, int part = 0);

@@ -70,7 +70,7 @@ struct StageBufs {
 
 // Split simulation (mixed chain, more than two rounds of one-lane waves): the one-lane kernel takes
 // the list slots of its full rounds, one 64-candidate wave per SIMD per round (cap SIMDs), and a
-// kernel with 4 lanes per candidate the rest, on another stream once the first has finished.
+// kernel with 2 lanes per candidate the rest, on another stream once the first has finished.
 // Without the split, the SIMDs that hold one wave more than the others (c3: 287 of 1,024 hold a
 // third) set the simulation's time while the rest wait.  Returns the first slot of the remainder,
 // or live (no split) with fewer than two full rounds or a remainder over half a round.

@@ -107,7 +107,7 @@ def test_mixed_two_streams_equal_one_stream(engine, name, n):
     every kernel runs in order on one stream with the plain simulation kernel.  Same rows, bit for bit.
     The full c3 block (149,427 simulated candidates: two full rounds of one-lane waves on the
     device's 1,024 SIMDs and a remainder of 287 waves) also splits the simulation: the remainder runs
-    with 4 lanes per candidate on a third stream, with its own fit."""
+    with 2 lanes per candidate on a third stream, with its own fit."""
     p = workloads.CONFIGS[name](n=n)
     print("case", name, n, flush=True)
     _, two = engine.evaluate(p, PRECISION_MIXED)
