@@ -34,11 +34,12 @@ WORKLOADS = {"c1": "mass-spring-damper 2-state", "c2": "two-mass drive 4-state, 
              "c3": "random stable 8-state, 4 NL, Kalman observer"}
 
 
-def kernel_flops(problem, info64, d, mixed):
+def kernel_flops(problem, info64, d, mixed, sim_slots=None):
     """Algorithmic flops per launch of each stage kernel, counted for the candidates that reach it
     (FMA = 2, add/mul/div = 1, each non-SKIP instruction = 1; transcendentals not counted).  The
     float simulation counts the compensated state of the reference (lo-part dot products and the
-    11-operation TwoSum / Fast2Sum update of every state)."""
+    11-operation TwoSum / Fast2Sum update of every state).  sim_slots: the candidates the simulation
+    kernel itself runs (the one-lane kernel of a split simulation), else every simulated one."""
     import numpy as np
     c = problem.config
     ns, nd, nw = int(c["statespace_dim"]), int(c["discrete_dim"]), int(c["nonlin_dim"])
@@ -57,7 +58,7 @@ def kernel_flops(problem, info64, d, mixed):
     per_step = 4 * ns + 5 + ns * (2 * ns + 4) + nd * (2 * nd + 2 * nw + 2) + ninstr
     if mixed:
         per_step += 4 * ns + 2 * ns * ns + 2 * nd * nd + 11 * (ns + nd)
-    sim = float(per_step) * T * n_sim
+    sim = float(per_step) * T * (n_sim if sim_slots is None else min(n_sim, sim_slots))
     solve = float((2 * nt) ** 3 / 3 + 4 * (2 * nt) ** 2) * int(fitted.sum())
     per_point = 8.0 / 3.0 * ns ** 3 + 8 * ns * ns + 8 * ns + 30
     swept = int((info64[:, d.info_allmargin + 3] == 1).sum())
@@ -296,6 +297,7 @@ def bench_block(args):
         dt = time.perf_counter() - t0
         sim_hist = eng.sim_history()
         sim_kernel = eng.last_sim_kernel()
+        split_cap = eng.last_sim_split()
         eng.set_sim_history(0)
         info64 = i64.cpu().numpy()
         # per-kernel durations from HIP events recorded on the launch stream, separate passes
@@ -308,12 +310,20 @@ def bench_block(args):
         eng.set_profiling(False)
         kern = {k: sum(v) / len(v) for k, v in acc.items() if k != "start"}
         results[name] = dict(value=n * args.steps / dt, ms=1000 * dt / args.steps, kernels=kern, info64=info64,
-                             sim_ms=sum(sim_hist) / len(sim_hist) if sim_hist else None, sim_kernel=sim_kernel)
+                             sim_ms=sum(sim_hist) / len(sim_hist) if sim_hist else None, sim_kernel=sim_kernel,
+                             split_cap=split_cap)
         print("%s %s: %.4g candidate-evals/s, %.3f ms per step" % (args.config, name, results[name]["value"],
                                                                    results[name]["ms"]), file=sys.stderr, flush=True)
     head = modes[0][0]
     r = results[head]
-    fl = kernel_flops(problem, r["info64"], d, head == "mixed")
+    # split simulation: the one-lane kernel (the roofline's) runs the list slots below the split,
+    # a 2-lane kernel the rest on another stream
+    from controllerengine_amd.engine import sim_split_slots
+    sc = r["info64"][:, d.info_scoring:d.info_scoring + 3]
+    n_sim = int((~((sc[:, 0] >= float(problem.config["unstable_err"])) &
+                   (r["info64"][:, d.info_allmargin + 3] == 0))).sum())
+    sim_slots = sim_split_slots(n_sim, r["split_cap"]) if r["split_cap"] else None
+    fl = kernel_flops(problem, r["info64"], d, head == "mixed", sim_slots)
     kern = r["kernels"]
     dom = max(kern, key=lambda k: kern[k]) if kern else None
     dom_ms = kern.get(dom)
@@ -344,7 +354,8 @@ def bench_block(args):
         "roofline": {"bound": "valu", "kernel": dom, "launch_ms": dom_ms, "launch_ms_source": ms_source,
                      "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": (achieved / peak) if achieved is not None else None,
-                     "traffic": traffic, "flops_per_launch": fl.get(dom)},
+                     "traffic": traffic, "flops_per_launch": fl.get(dom),
+                     "simulated": n_sim, "kernel_candidates": sim_slots if sim_slots is not None else n_sim},
     }
     if "fp64" in results and head != "fp64":
         line["fp64_value"] = results["fp64"]["value"]

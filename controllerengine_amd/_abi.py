@@ -69,7 +69,7 @@ EXPORTS = [
     "RunControllerEngine", "ce_config_parse", "ce_config_derive", "ce_plant_load_dir", "ce_engine_create",
     "ce_engine_destroy", "ce_evaluate", "ce_engine_set_diagnostics", "ce_engine_stage_times",
     "ce_engine_last_sim_jit", "ce_engine_jit_compiles", "ce_engine_set_sim_history", "ce_engine_sim_history",
-    "ce_engine_last_sim_kernel", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_engine_sweep_counts",
+    "ce_engine_last_sim_kernel", "ce_engine_last_sim_split", "ce_sim_split_slots", "ce_engine_set_profiling", "ce_engine_kernel_times", "ce_engine_sweep_counts",
     "ce_detmath_host",
     "ce_detmath_device", "ce_batch_create", "ce_batch_destroy", "ce_batch_clear", "ce_batch_add", "ce_batch_update",
     "ce_batch_offsets", "ce_batch_set_options", "ce_batch_set_plan", "ce_batch_plan", "ce_batch_run", "ce_batch_info", "ce_batch_kernel_times",
@@ -108,6 +108,10 @@ def _declare(lib):
     lib.ce_engine_jit_compiles.restype = ctypes.c_longlong
     lib.ce_engine_last_sim_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     lib.ce_engine_last_sim_kernel.restype = ctypes.c_int
+    lib.ce_engine_last_sim_split.argtypes = [vp]
+    lib.ce_engine_last_sim_split.restype = ctypes.c_int
+    lib.ce_sim_split_slots.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.ce_sim_split_slots.restype = ctypes.c_int
     lib.ce_engine_set_sim_history.argtypes = [vp, ctypes.c_int]
     lib.ce_engine_set_sim_history.restype = ctypes.c_int
     lib.ce_engine_sim_history.argtypes = [vp, P(ctypes.c_double), ctypes.c_int]

@@ -104,6 +104,7 @@ struct ce_engine {
   // split simulation (sim_split_slots): the remainder and its fit on a third stream
   hipStream_t rem = nullptr;
   hipEvent_t ev_sima = nullptr, ev_rem = nullptr;
+  int last_split_cap = 0;  // the last call's split simulation: the SIMD count of its rule, 0 = no split
   // start signal of the gated simulation (one int, value = gate_gen of the last gated launch)
   Buf gate;
   int gate_gen = 0;
@@ -409,9 +410,11 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
     E->last_sim_name = mixed ? "k_sim_f32" : "k_sim_f64";
     if (!ok) throw std::runtime_error("no simulation kernel for these dimensions");
   }
+  E->last_split_cap = split ? E->n_simd : 0;
   if (hslot >= 0) {
-    // the stage ends with the remainder when the simulation is split
-    HIPCHK(hipEventRecord(E->hist_b[hslot], split ? E->rem : st));
+    // split simulation: the events bracket the one-lane kernel (the remainder follows on the third
+    // stream; the slots of each part are given by ce_sim_split_slots)
+    HIPCHK(hipEventRecord(E->hist_b[hslot], st));
     E->hist_pos = (hslot + 1) % (int)E->hist_a.size();
     E->hist_n = std::min(E->hist_n + 1, (int)E->hist_a.size());
   }
@@ -563,6 +566,10 @@ extern "C" CE_EXPORT int ce_engine_last_sim_kernel(ce_engine* E, char* name, int
   std::snprintf(name, (size_t)len, "%s", E->last_sim_name);
   return 0;
 }
+
+extern "C" CE_EXPORT int ce_engine_last_sim_split(ce_engine* E) { return E ? E->last_split_cap : 0; }
+
+extern "C" CE_EXPORT int ce_sim_split_slots(int live, int cap) { return sim_split_slots(live, cap); }
 
 // Simulation-stage history: `calls` event pairs (0 turns it off); durations of the recorded calls.
 extern "C" CE_EXPORT int ce_engine_set_sim_history(ce_engine* E, int calls) {

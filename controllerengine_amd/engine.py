@@ -114,6 +114,11 @@ class Engine:
         _abi.lib().ce_engine_last_sim_kernel(self.h, buf, 64)
         return buf.value.decode()
 
+    def last_sim_split(self):
+        """SIMD count the last call split its simulation by (0: not split); the one-lane kernel took
+        the list slots below sim_split_slots(live, cap), a 2-lane kernel the rest."""
+        return int(_abi.lib().ce_engine_last_sim_split(self.h))
+
     def set_sim_history(self, calls):
         """Record the simulation kernel's duration (HIP events on the launch stream) for the next
         `calls` evaluations (a ring; 0 turns it off)."""
@@ -150,3 +155,9 @@ class Engine:
         if k <= 0:
             return []
         return list(zip(names.value.decode().split(","), [buf[i] for i in range(k)]))
+
+
+def sim_split_slots(live, cap):
+    """First list slot of the split simulation's remainder for `live` simulated candidates and `cap`
+    SIMDs (`live` when the simulation is not split)."""
+    return int(_abi.lib().ce_sim_split_slots(int(live), int(cap)))

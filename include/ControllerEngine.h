@@ -157,6 +157,13 @@ CE_EXPORT int ce_engine_stage_times(ce_engine* eng, double* ms, int n_max);
 CE_EXPORT int ce_engine_last_sim_jit(ce_engine* eng);
 /* Kernel name of the last call's simulation launch (as a kernel trace lists it). */
 CE_EXPORT int ce_engine_last_sim_kernel(ce_engine* eng, char* name, int len);
+/* Split simulation (no counterpart in the reference): when the mixed chain's one-lane simulation
+ * has more than two full rounds of one wave per SIMD, the one-lane kernel takes the list slots of
+ * the full rounds and a 2-lane kernel the rest, on another stream.  ce_engine_last_sim_split
+ * returns the SIMD count the last call split by (0: not split); ce_sim_split_slots(live, cap) the
+ * first list slot of the remainder for live simulated candidates (live when there is none). */
+CE_EXPORT int ce_engine_last_sim_split(ce_engine* eng);
+CE_EXPORT int ce_sim_split_slots(int live, int cap);
 /* hipRTC compilations this engine has done (no counterpart in the reference).  A problem is
  * compiled with its constants written into the source; once the engine sees a second problem of
  * the same dimensions and instruction structure (the rounds of a serial-mode run, a search over
