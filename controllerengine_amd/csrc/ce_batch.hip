@@ -287,7 +287,7 @@ static void build_layout(ce_batch* b, int precision, hipStream_t st) {
     s.list = seg(n * 4);
     s.nlist = seg(n * 4);
     if (mixed) {
-      s.mr32 = seg(n * (size_t)p.F * 4 * 4);
+      s.mr32 = seg(n * (size_t)((p.F + 63) / 64) * 64 * 4 * 4);
       s.refined = seg(((n * (size_t)p.F + 3) / 4) * 4);
       s.rlist = seg(n * RMAX * sizeof(int2));
       s.flist = seg(n * 4);

@@ -72,8 +72,25 @@ struct ProblemDev {
   float fts, fref_amp;
   // byte offsets (from the start of this struct) of the per-frequency tables
   int64_t off_zr, off_zi, off_theta, off_pr, off_pi, off_pabs, off_pok;
-  int64_t off_zm1;  // Re(z) - 1 per frequency (double), for the mixed sweep's delta form
+  // The mixed sweep's per-frequency inputs and the grid angles in lane-interleaved order: a wave
+  // gives each lane KM = ceil(F / 64) consecutive frequencies, and frequency w = l * KM + k is
+  // stored at slot k * 64 + l, so that the loads of one k by the 64 lanes are consecutive.
+  int64_t off_sweep_il;  // SweepPointIl[KM * 64]
+  int64_t off_theta_il;  // double[KM * 64]: theta_w
   int64_t bytes_total;
 };
+
+// One frequency of the mixed sweep (lane-interleaved table): z - 1 in double (its real part
+// exact in the delta form), the plant response rounded to float as the sweep uses it, and whether
+// the plant response is finite.
+struct alignas(16) SweepPointIl {
+  double zm1, zi;
+  float pr, pi, pabs;
+  int32_t ok;
+};
+static_assert(sizeof(SweepPointIl) == 32, "two 16-byte loads per frequency");
+
+// slot of frequency w in a lane-interleaved table with km frequencies per lane
+__host__ __device__ inline int il_slot(int w, int km) { return (w % km) * 64 + w / km; }
 
 }  // namespace ce

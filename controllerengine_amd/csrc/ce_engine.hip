@@ -226,7 +226,7 @@ static void evaluate(ce_engine* E, const ce_config& cfg, const ce_plant& plant, 
   B.count = (int32_t*)ensure(E->count, 2 * sizeof(int32_t));
   B.ncount = B.count + 1;
   if (mixed) {
-    B.mr32 = (float*)ensure(E->mr32, nn * (size_t)F * 4 * sizeof(float));
+    B.mr32 = (float*)ensure(E->mr32, nn * (size_t)((F + 63) / 64) * 64 * 4 * sizeof(float));
     B.refined = (int8_t*)ensure(E->refined, ((nn * (size_t)F + 3) / 4) * 4);
     B.rlist = (int2*)ensure(E->rlist, nn * RMAX * sizeof(int2));
     B.rcount = (int32_t*)ensure(E->rcount, sizeof(int32_t));

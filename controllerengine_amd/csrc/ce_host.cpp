@@ -404,7 +404,11 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   const int F = c.freq;
   const size_t head = (sizeof(ProblemDev) + 255) & ~(size_t)255;
   const size_t tab = ((size_t)F * 8 + 255) & ~(size_t)255;
-  blob.assign(head + 8 * tab, 0);
+  const int km = (F + 63) / 64;
+  const size_t slots = (size_t)km * 64;
+  const size_t tab_sw = (slots * sizeof(SweepPointIl) + 255) & ~(size_t)255;
+  const size_t tab_th = (slots * 8 + 255) & ~(size_t)255;
+  blob.assign(head + 7 * tab + tab_sw + tab_th, 0);
   ProblemDev& P = *reinterpret_cast<ProblemDev*>(blob.data());
   P.ns = c.statespace_dim; P.nd = c.discrete_dim; P.nt = P.ns + P.nd; P.nw = c.nonlin_dim;
   P.T = c.time; P.F = F;
@@ -470,7 +474,8 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   P.off_pi = P.off_pr + (int64_t)tab;
   P.off_pabs = P.off_pi + (int64_t)tab;
   P.off_pok = P.off_pabs + (int64_t)tab;
-  P.off_zm1 = P.off_pok + (int64_t)tab;
+  P.off_sweep_il = P.off_pok + (int64_t)tab;
+  P.off_theta_il = P.off_sweep_il + (int64_t)tab_sw;
   P.bytes_total = (int64_t)blob.size();
   double* zr = reinterpret_cast<double*>(blob.data() + P.off_zr);
   double* zi = reinterpret_cast<double*>(blob.data() + P.off_zi);
@@ -479,7 +484,8 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
   double* pim = reinterpret_cast<double*>(blob.data() + P.off_pi);
   double* pab = reinterpret_cast<double*>(blob.data() + P.off_pabs);
   int32_t* pok = reinterpret_cast<int32_t*>(blob.data() + P.off_pok);
-  double* zm1 = reinterpret_cast<double*>(blob.data() + P.off_zm1);
+  SweepPointIl* swi = reinterpret_cast<SweepPointIl*>(blob.data() + P.off_sweep_il);
+  double* thi = reinterpret_cast<double*>(blob.data() + P.off_theta_il);
   // log-spaced grid from freq_min to Nyquist, derived in float32 as the reference does
   // (src/main.cu:423-427: logf/expf spacing, z = (cosf, sinf) of the float angle) and promoted;
   // theta_w = arg(z) of the float point (src/allmargin.cu:111)
@@ -492,9 +498,6 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
     const double co = cf, s = sf;
     zr[w] = co;
     zi[w] = s;
-    // Re(z) - 1: exact in double for a float Re(z) in [0.5, 1] (Sterbenz), and the difference of
-    // two doubles otherwise
-    zm1[w] = co - 1.0;
     th[w] = (double)cedm::atan2f_(sf, cf);
     double xr[MAXS], xi[MAXS];
     const bool ok = shifted_solve_host(P.pa, ns, co, s, P.pb, xr, xi);
@@ -509,6 +512,16 @@ bool build_problem(const ce_config& c, const ce_plant& pl, std::vector<uint8_t>&
       pim[w] = 0.0;
     }
     pab[w] = cedm::hypot(pr[w], pim[w]);
+    SweepPointIl& q = swi[il_slot(w, km)];
+    // Re(z) - 1: exact in double for a float Re(z) in [0.5, 1] (Sterbenz), and the difference of
+    // two doubles otherwise
+    q.zm1 = co - 1.0;
+    q.zi = s;
+    q.pr = (float)pr[w];
+    q.pi = (float)pim[w];
+    q.pabs = (float)pab[w];
+    q.ok = pok[w];
+    thi[il_slot(w, km)] = th[w];
   }
   {  // alpha: reference src/allmargin.cu:32-41, at z = polar(1, crossover * ts) in float
     float sf, cf;

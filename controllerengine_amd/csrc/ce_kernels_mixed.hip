@@ -155,15 +155,15 @@ __global__ __launch_bounds__(64) void k_atb(const ProblemDev* __restrict__ P, St
 }
 
 // local block -> (candidate block, mode block, time part) of the problem's own grid
+template <bool LOOP>
 __global__ __launch_bounds__(64) void k_atb_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     const ProblemDev* P = T.probs[q.p];
     const StageBufs& B = T.bufs[q.p];
     const int nbx = (B.n + 63) / 64, nby = (P->nt + ATB_MODES - 1) / ATB_MODES;
     const int r = q.lb / nbx;
     k_atb_body(P, B, q.lb % nbx, nbx, r % nby, r / nby);
-  }
+  });
 }
 
 
@@ -312,12 +312,9 @@ __global__ __launch_bounds__(64) void k_fitcf(const ProblemDev* __restrict__ P, 
   k_fitcf_body<NB>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int NB>
+template <int NB, bool LOOP>
 __global__ __launch_bounds__(64) void k_fitcf_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
-    k_fitcf_body<NB>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) { k_fitcf_body<NB>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0); });
 }
 
 
@@ -366,11 +363,12 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   const int F = P->F, lane = threadIdx.x;
   const int KM = (F + 63) / 64;
   const int w0 = lane * KM;
-  const int32_t* pok = tab<int32_t>(P, P->off_pok);
   const double* oa = B.open_a + (size_t)c * NS * NS;
   const float* gg = B.gains + (size_t)c * P->cuda_w;
   bool failp = false;
-  float* mr = B.mr32 + (size_t)c * F * 4;
+  // the located values in the frequency tables' lane-interleaved order (point k of this lane at
+  // slot k * 64 + lane, one 16-byte store per point; the 64 lanes' stores of one k are consecutive)
+  float4* mr = reinterpret_cast<float4*>(B.mr32) + (size_t)c * KM * 64 + lane;
   const int nk = (w0 < F) ? ((F - w0 < KM) ? F - w0 : KM) : 0;  // points of this lane
   // The Hessenberg form is the same for every lane: it is reduced and kept in LDS (H, Q^T K,
   // Q^T L) and read by broadcast, which leaves the registers to the point solves.
@@ -378,30 +376,21 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
   double hnorm;
   __syncthreads();  // a batched launch reuses sh for its next candidate
   hess_prep_lds<NS>(oa, gg, sh, hnorm);
-  const double* zm1t = tab<double>(P, P->off_zm1);
-  const double* zit = tab<double>(P, P->off_zi);
-  const double* prt = tab<double>(P, P->off_pr);
-  const double* pit = tab<double>(P, P->off_pi);
-  const double* pat = tab<double>(P, P->off_pabs);
+  // point k of this lane: slot k * 64 + lane of the lane-interleaved table
+  const SweepPointIl* spt = tab<SweepPointIl>(P, P->off_sweep_il) + lane;
   const float alpha = (float)P->alpha;
   // the sweep points go to memory and are read back below, so that nothing else is live across
   // the register-heavy elimination; each point's table values are loaded one point ahead
-  double nzr = 0.0, nzi = 0.0, npr = 0.0, npi = 0.0, npa = 0.0;
-  int npok = 0;
-  if (nk > 0) {
-    nzr = zm1t[w0]; nzi = zit[w0]; npr = prt[w0]; npi = pit[w0]; npa = pat[w0]; npok = pok[w0];
-  }
+  SweepPointIl nq{};
+  if (nk > 0) nq = spt[0];
 #pragma unroll 1
   for (int k = 0; k < nk; k++) {
-    const int w = w0 + k;
     // keeps the LDS reads of the Hessenberg data inside the loop (not hoisted into registers)
     __asm__ volatile("" ::: "memory");
-    const double zr = nzr, zi = nzi;
-    const float pr = (float)npr, pim = (float)npi, pa = (float)npa;
-    const int ok = npok;
-    if (k + 1 < nk) {
-      nzr = zm1t[w + 1]; nzi = zit[w + 1]; npr = prt[w + 1]; npi = pit[w + 1]; npa = pat[w + 1]; npok = pok[w + 1];
-    }
+    const double zr = nq.zm1, zi = nq.zi;
+    const float pr = nq.pr, pim = nq.pi, pa = nq.pabs;
+    const int ok = nq.ok;
+    if (k + 1 < nk) nq = spt[(k + 1) * 64];
     float mm = __builtin_nanf(""), rr = __builtin_nanf(""), b2 = __builtin_nanf(""), e2 = 1.0f;
     if (!ok) {
       failp = true;
@@ -409,10 +398,7 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
       float m1, r1, bq, e1;
       if (hess_point<NS>(sh, hnorm, zr, zi, pr, pim, pa, alpha, m1, r1, bq, e1)) { mm = m1; rr = r1; b2 = bq; e2 = e1; }
     }
-    mr[4 * w] = mm;
-    mr[4 * w + 1] = rr;
-    mr[4 * w + 2] = b2;
-    mr[4 * w + 3] = e2;
+    mr[k * 64] = make_float4(mm, rr, b2, e2);
   }
   if (__ballot(failp)) {
     if (lane == 0) B.sweep_fail[c] = 1;
@@ -424,11 +410,11 @@ __device__ __forceinline__ void k_sweep_mark32_body(const ProblemDev* __restrict
     m[k] = raw[k] = bb[k] = __builtin_nanf("");
     er[k] = 1.0f;
     if (k < nk) {
-      const int w = w0 + k;
-      m[k] = mr[4 * w];
-      raw[k] = mr[4 * w + 1];
-      bb[k] = mr[4 * w + 2];
-      er[k] = mr[4 * w + 3];
+      const float4 v = mr[k * 64];
+      m[k] = v.x;
+      raw[k] = v.y;
+      bb[k] = v.z;
+      er[k] = v.w;
     }
   }
   const float PIf = (float)PI_D, TPf = (float)TWO_PI_D;
@@ -579,12 +565,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   k_sweep_mark32_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int NS>
+template <int NS, bool LOOP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_mark32_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
-    k_sweep_mark32_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) { k_sweep_mark32_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0); });
 }
 
 
@@ -627,12 +610,9 @@ __device__ __forceinline__ void k_full64_body(const ProblemDev* __restrict__ P, 
   }
 }
 
-template <int NS>
+template <int NS, bool LOOP>
 __global__ __launch_bounds__(64) void k_full64_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
-    k_full64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) { k_full64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0); });
 }
 
 // The point refinement (first REFINE_BLOCKS workgroups) and the whole-candidate FP64 sweeps (the
@@ -645,12 +625,9 @@ __global__ __launch_bounds__(64) void k_refine_full64(const ProblemDev* __restri
   else k_full64_body<NS>(P, B, bx - REFINE_BLOCKS, FULL64_BLOCKS, 0, 0);
 }
 
-template <int NS>
+template <int NS, bool LOOP>
 __global__ __launch_bounds__(64) void k_refine64_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
-    k_refine64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) { k_refine64_body<NS>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0); });
 }
 
 
@@ -709,7 +686,7 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
   double m[KMAX], raw[KMAX], bb[KMAX];
   bool failp = false;
   if (!fail) {
-    const float* mr = B.mr32 + (size_t)c * F * 4;
+    const float4* mr = reinterpret_cast<const float4*>(B.mr32) + (size_t)c * KM * 64 + lane;
     const double* m64 = B.mag_raw + (size_t)c * F * 3;
     const int8_t* rf = B.refined + (size_t)c * F;
     // the lane's float values and refinement marks are loaded together, ahead of their use
@@ -722,20 +699,32 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
       if (k < nk) {
         const int w = w0 + k;
         rv[k] = rf[w];
-        fv[k][0] = mr[4 * w];
-        fv[k][1] = mr[4 * w + 1];
-        fv[k][2] = mr[4 * w + 2];
+        const float4 v = mr[k * 64];
+        fv[k][0] = v.x;
+        fv[k][1] = v.y;
+        fv[k][2] = v.z;
       }
     }
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
       m[k] = raw[k] = bb[k] = 0.0;
+      if (k < nk) { m[k] = (double)fv[k][0]; raw[k] = (double)fv[k][1]; bb[k] = (double)fv[k][2]; }
+    }
+    // the FP64 values of the refined points: every load is issued before any is used, so a wave
+    // waits once for all of them
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      if (k < nk && rv[k] == 2) {
+        const int w = w0 + k;
+        m[k] = m64[3 * w];
+        raw[k] = m64[3 * w + 1];
+        bb[k] = m64[3 * w + 2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
       if (k >= nk) continue;
-      const int w = w0 + k;
-      const int r = rv[k];
-      if (r == 3) failp = true;
-      if (r == 2) { m[k] = m64[3 * w]; raw[k] = m64[3 * w + 1]; bb[k] = m64[3 * w + 2]; }
-      else { m[k] = (double)fv[k][0]; raw[k] = (double)fv[k][1]; bb[k] = (double)fv[k][2]; }
+      if (rv[k] == 3) failp = true;
       if (!(cedm::isfinite_(m[k]) && cedm::isfinite_(raw[k]))) failp = true;
     }
   }
@@ -751,7 +740,8 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
     return;
   }
   const double ts = P->ts;
-  const double* th = tab<double>(P, P->off_theta);
+  // grid angles, lane-interleaved: theta of point k of this lane at slot k * 64 + lane
+  const double* thi = tab<double>(P, P->off_theta_il);
   double Bmin = 1.0;
   double lastRaw = raw[0], lastM = m[0];
 #pragma unroll
@@ -799,7 +789,9 @@ __device__ __forceinline__ void k_scan32_body(const ProblemDev* __restrict__ P, 
     if (w == 0) continue;
     const double prevMag = (k == 0) ? prevM0 : m[k > 0 ? k - 1 : 0];
     const double prevPhase = (k == 0) ? prevPh0 : ph[k > 0 ? k - 1 : 0];
-    const double prevTheta = th[w - 1], theta_w = th[w];
+    // theta_{w-1}: the previous point of this lane, or (k = 0, lane > 0) the last of the lane below
+    const double prevTheta = (k == 0) ? thi[(KM - 1) * 64 + lane - 1] : thi[(k - 1) * 64 + lane];
+    const double theta_w = thi[k * 64 + lane];
     const double mk = m[k], phk = ph[k];
     if ((prevMag - 1.0) * (mk - 1.0) <= 0.0 && mk != prevMag) {
       const double frac = (1.0 - prevMag) / (mk - prevMag);
@@ -888,11 +880,9 @@ __global__ __launch_bounds__(64) void k_scan32_split(const ProblemDev* __restric
   k_scan32_body<true>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
+template <bool LOOP>
 __global__ __launch_bounds__(64) void k_scan32_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
-    k_scan32_body<false>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) { k_scan32_body<false>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0); });
 }
 
 
@@ -971,36 +961,44 @@ void launch_margin_combine_final(const ProblemDev* P, const StageBufs& B, float*
   kmark("k_margin_combine_final", st);
 }
 
-bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
-  const dim3 g(grid), b(64);
+// A grid of a workgroup per block launches the kernels' loop-free form (fewer registers).
+template <bool LOOP>
+static bool launch_batch_mixed_t(int kind, int key, const BatchTab& T, const dim3 g, hipStream_t st) {
+  const dim3 b(64);
 #define CE_NS_CASES(K)                                                      \
   switch (key) {                                                            \
-    case 1: hipLaunchKernelGGL(K<1>, g, b, 0, st, T); return true;          \
-    case 2: hipLaunchKernelGGL(K<2>, g, b, 0, st, T); return true;          \
-    case 3: hipLaunchKernelGGL(K<3>, g, b, 0, st, T); return true;          \
-    case 4: hipLaunchKernelGGL(K<4>, g, b, 0, st, T); return true;          \
-    case 5: hipLaunchKernelGGL(K<5>, g, b, 0, st, T); return true;          \
-    case 6: hipLaunchKernelGGL(K<6>, g, b, 0, st, T); return true;          \
-    case 7: hipLaunchKernelGGL(K<7>, g, b, 0, st, T); return true;          \
-    case 8: hipLaunchKernelGGL(K<8>, g, b, 0, st, T); return true;          \
+    case 1: hipLaunchKernelGGL((K<1, LOOP>), g, b, 0, st, T); return true;  \
+    case 2: hipLaunchKernelGGL((K<2, LOOP>), g, b, 0, st, T); return true;  \
+    case 3: hipLaunchKernelGGL((K<3, LOOP>), g, b, 0, st, T); return true;  \
+    case 4: hipLaunchKernelGGL((K<4, LOOP>), g, b, 0, st, T); return true;  \
+    case 5: hipLaunchKernelGGL((K<5, LOOP>), g, b, 0, st, T); return true;  \
+    case 6: hipLaunchKernelGGL((K<6, LOOP>), g, b, 0, st, T); return true;  \
+    case 7: hipLaunchKernelGGL((K<7, LOOP>), g, b, 0, st, T); return true;  \
+    case 8: hipLaunchKernelGGL((K<8, LOOP>), g, b, 0, st, T); return true;  \
   }                                                                         \
   return false;
   switch (kind) {
-    case BK_ATB: hipLaunchKernelGGL(k_atb_batch, g, b, 0, st, T); return true;
+    case BK_ATB: hipLaunchKernelGGL(k_atb_batch<LOOP>, g, b, 0, st, T); return true;
     case BK_FITCF:
       switch (key) {
-        case 1: hipLaunchKernelGGL(k_fitcf_batch<1>, g, b, 0, st, T); return true;
-        case 2: hipLaunchKernelGGL(k_fitcf_batch<2>, g, b, 0, st, T); return true;
-        case 3: hipLaunchKernelGGL(k_fitcf_batch<3>, g, b, 0, st, T); return true;
-        case 4: hipLaunchKernelGGL(k_fitcf_batch<4>, g, b, 0, st, T); return true;
+        case 1: hipLaunchKernelGGL((k_fitcf_batch<1, LOOP>), g, b, 0, st, T); return true;
+        case 2: hipLaunchKernelGGL((k_fitcf_batch<2, LOOP>), g, b, 0, st, T); return true;
+        case 3: hipLaunchKernelGGL((k_fitcf_batch<3, LOOP>), g, b, 0, st, T); return true;
+        case 4: hipLaunchKernelGGL((k_fitcf_batch<4, LOOP>), g, b, 0, st, T); return true;
       }
       return false;
     case BK_SWEEP_MARK32: CE_NS_CASES(k_sweep_mark32_batch)
     case BK_REFINE64: CE_NS_CASES(k_refine64_batch)
     case BK_FULL64: CE_NS_CASES(k_full64_batch)
-    case BK_SCAN32: hipLaunchKernelGGL(k_scan32_batch, g, b, 0, st, T); return true;
+    case BK_SCAN32: hipLaunchKernelGGL(k_scan32_batch<LOOP>, g, b, 0, st, T); return true;
   }
   return false;
+#undef CE_NS_CASES
+}
+
+bool launch_batch_mixed(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
+  return grid >= T.total ? launch_batch_mixed_t<false>(kind, key, T, dim3(grid), st)
+                         : launch_batch_mixed_t<true>(kind, key, T, dim3(grid), st);
 }
 
 bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, hipStream_t st);

@@ -109,7 +109,7 @@ size_t footprint(const ce_config& c, int n, int precision) {
   per += (size_t)d.cuda_mx_width * 4 + (size_t)d.info_mx_width * (4 + 4 + 8 + 8);  // in, out, out64, info64
   per += 4 * 5 + 2 * ce::MAXT * 8 + (size_t)c.statespace_dim * c.statespace_dim * 8;
   per += fc * (mixed ? 4 : 8) + F * (mixed ? 3 : 2) * 8;
-  if (mixed) per += F * 4 * 4 + F + ce::RMAX * 8 + 4 + 16 * nt * 16;
+  if (mixed) per += (F + 63) / 64 * 64 * 4 * 4 + F + ce::RMAX * 8 + 4 + 16 * nt * 16;
   return per * (size_t)n + 64 * 1024;
 }
 

@@ -45,7 +45,8 @@ struct StageBufs {
   int32_t* nlist;               // n: candidates whose nominal loop needs its own eigensolve
   int32_t* ncount;              // 1: entries in nlist
   // mixed sweep
-  float* mr32;                  // n x F x 4: located |loop|, arg(loop), b(w), error estimate (float)
+  float* mr32;                  // n x (KM * 64) x 4, frequency w at slot il_slot(w, KM): located |loop|,
+                                // arg(loop), b(w), error estimate (float)
   int8_t* refined;              // n x F: 0 float value, 1 marked, 2 refined in FP64, 3 FP64 not finite
   int2* rlist;                  // n x RMAX: dense (candidate, frequency) list
   int32_t* rcount;              // 1
@@ -141,6 +142,18 @@ __device__ __forceinline__ BatchSlot batch_slot(const BatchTab& T, int b) {
   const int s0 = __builtin_amdgcn_readfirstlane(T.start[lo]);
   const int s1 = __builtin_amdgcn_readfirstlane(T.start[lo + 1]);
   return BatchSlot{p, b - s0, s1 - s0};
+}
+
+// The blocks of a batched launch.  LOOP: the workgroups stride over the blocks (a persistent launch
+// on a capped grid).  Otherwise the launch has a workgroup per block and the kernel holds no block
+// loop, which costs registers (the loop keeps values of the body live across its iterations).
+template <bool LOOP, typename Body>
+__device__ __forceinline__ void batch_blocks(const BatchTab& T, Body&& body) {
+  if constexpr (LOOP) {
+    for (int b = blockIdx.x; b < T.total; b += gridDim.x) body(batch_slot(T, b));
+  } else {
+    if ((int)blockIdx.x < T.total) body(batch_slot(T, (int)blockIdx.x));
+  }
 }
 
 // Optional per-kernel timing.  Launchers call kmark(name, stream) after each stage kernel; when
