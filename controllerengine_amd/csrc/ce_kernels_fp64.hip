@@ -67,11 +67,11 @@ __global__ __launch_bounds__(64) void k_compact(const ProblemDev* __restrict__ P
   k_compact_body(P, B, blockIdx.x, gridDim.x, 0, 0);
 }
 
+template <bool LOOP>
 __global__ __launch_bounds__(64) void k_compact_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     k_compact_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  });
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -209,13 +209,12 @@ __global__ __launch_bounds__(64) void k_eig_lin(const ProblemDev* __restrict__ P
   k_eig_lin_body<G, R>(lds, P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int G, int R>
+template <int G, int R, bool LOOP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_lin_batch(BatchTab T) {
   __shared__ double lds[eig_lds_doubles<G, R>()];
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     k_eig_lin_body<G, R>(lds, T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  });
 }
 
 
@@ -265,13 +264,12 @@ __global__ __launch_bounds__(64) void k_eig_nom(const ProblemDev* __restrict__ P
   k_eig_nom_body<G, R>(lds, P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int G, int R>
+template <int G, int R, bool LOOP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_nom_batch(BatchTab T) {
   __shared__ double lds[eig_lds_doubles<G, R>()];
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     k_eig_nom_body<G, R>(lds, T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  });
 }
 
 // One launch for every problem of the stage, whatever its order: each block runs the default tile
@@ -282,11 +280,10 @@ constexpr int EIG_FUSED_LDS = eig_lds_doubles<4, 1>() > eig_lds_doubles<EIG8_G, 
                                   : (eig_lds_doubles<EIG8_G, 8 / EIG8_G>() > eig_lds_doubles<EIG16_G, 16 / EIG16_G>()
                                          ? eig_lds_doubles<EIG8_G, 8 / EIG8_G>() : eig_lds_doubles<EIG16_G, 16 / EIG16_G>());
 
-template <bool NOM>
+template <bool NOM, bool LOOP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_eig_batch_fused(BatchTab T) {
   __shared__ double lds[EIG_FUSED_LDS];
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     const ProblemDev* P = T.probs[q.p];
     const int key = eig_key(NOM ? 2 * P->ns : P->nt);
     if (key == 4) {
@@ -299,7 +296,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       if (NOM) k_eig_nom_body<EIG16_G, 16 / EIG16_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
       else k_eig_lin_body<EIG16_G, 16 / EIG16_G>(lds, P, T.bufs[q.p], q.lb, q.nb, 0, 0);
     }
-  }
+  });
 }
 #undef CE_ROWS
 
@@ -504,12 +501,11 @@ __global__ __launch_bounds__(64) void k_fit_fp64(const ProblemDev* __restrict__ 
   k_fit_fp64_body<NB>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int NB>
+template <int NB, bool LOOP>
 __global__ __launch_bounds__(64) void k_fit_fp64_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     k_fit_fp64_body<NB>(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  });
 }
 
 
@@ -538,13 +534,12 @@ __global__ __launch_bounds__(64) void k_sweep_fp64(const ProblemDev* __restrict_
   k_sweep_fp64_body<NS>(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
-template <int NS>
+template <int NS, bool LOOP>
 __global__ __launch_bounds__(64) void k_sweep_fp64_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     const StageBufs& B = T.bufs[q.p];
     k_sweep_fp64_body<NS>(T.probs[q.p], B, q.lb % B.n, B.n, q.lb / B.n, 0);
-  }
+  });
 }
 
 
@@ -632,11 +627,11 @@ __global__ __launch_bounds__(64) void k_scan_fp64(const ProblemDev* __restrict__
   k_scan_fp64_body(P, B, blockIdx.x, gridDim.x, blockIdx.y, blockIdx.z);
 }
 
+template <bool LOOP>
 __global__ __launch_bounds__(64) void k_scan_fp64_batch(BatchTab T) {
-  for (int b = blockIdx.x; b < T.total; b += gridDim.x) {
-    const BatchSlot q = batch_slot(T, b);
+  batch_blocks<LOOP>(T, [&](const BatchSlot& q) {
     k_scan_fp64_body(T.probs[q.p], T.bufs[q.p], q.lb, q.nb, 0, 0);
-  }
+  });
 }
 
 
@@ -702,55 +697,63 @@ bool launch_sweep_fp64(int ns, int F, const ProblemDev* P, const StageBufs& B, h
   return true;
 }
 
-bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
-  const dim3 g(grid), b(64);
+// A grid of a workgroup per block launches the kernels' loop-free form (fewer registers).
+template <bool LOOP>
+static bool launch_batch_fp64_t(int kind, int key, const BatchTab& T, const dim3 g, hipStream_t st) {
+  const dim3 b(64);
 #define CE_NS_CASES(K)                                                      \
   switch (key) {                                                            \
-    case 1: hipLaunchKernelGGL(K<1>, g, b, 0, st, T); return true;          \
-    case 2: hipLaunchKernelGGL(K<2>, g, b, 0, st, T); return true;          \
-    case 3: hipLaunchKernelGGL(K<3>, g, b, 0, st, T); return true;          \
-    case 4: hipLaunchKernelGGL(K<4>, g, b, 0, st, T); return true;          \
-    case 5: hipLaunchKernelGGL(K<5>, g, b, 0, st, T); return true;          \
-    case 6: hipLaunchKernelGGL(K<6>, g, b, 0, st, T); return true;          \
-    case 7: hipLaunchKernelGGL(K<7>, g, b, 0, st, T); return true;          \
-    case 8: hipLaunchKernelGGL(K<8>, g, b, 0, st, T); return true;          \
+    case 1: hipLaunchKernelGGL((K<1, LOOP>), g, b, 0, st, T); return true;          \
+    case 2: hipLaunchKernelGGL((K<2, LOOP>), g, b, 0, st, T); return true;          \
+    case 3: hipLaunchKernelGGL((K<3, LOOP>), g, b, 0, st, T); return true;          \
+    case 4: hipLaunchKernelGGL((K<4, LOOP>), g, b, 0, st, T); return true;          \
+    case 5: hipLaunchKernelGGL((K<5, LOOP>), g, b, 0, st, T); return true;          \
+    case 6: hipLaunchKernelGGL((K<6, LOOP>), g, b, 0, st, T); return true;          \
+    case 7: hipLaunchKernelGGL((K<7, LOOP>), g, b, 0, st, T); return true;          \
+    case 8: hipLaunchKernelGGL((K<8, LOOP>), g, b, 0, st, T); return true;          \
   }                                                                         \
   return false;
   switch (kind) {
     // eigen keys: 0 = the fused launch over every order; else capacity * 100 + lanes per candidate
     case BK_EIG_LIN:
       switch (key) {
-        case 0: hipLaunchKernelGGL(k_eig_batch_fused<false>, g, b, 0, st, T); return true;
-        case 404: hipLaunchKernelGGL((k_eig_lin_batch<4, 1>), g, b, 0, st, T); return true;
-        case 804: hipLaunchKernelGGL((k_eig_lin_batch<4, 2>), g, b, 0, st, T); return true;
-        case 808: hipLaunchKernelGGL((k_eig_lin_batch<8, 1>), g, b, 0, st, T); return true;
-        case 1608: hipLaunchKernelGGL((k_eig_lin_batch<8, 2>), g, b, 0, st, T); return true;
-        case 1616: hipLaunchKernelGGL((k_eig_lin_batch<16, 1>), g, b, 0, st, T); return true;
+        case 0: hipLaunchKernelGGL((k_eig_batch_fused<false, LOOP>), g, b, 0, st, T); return true;
+        case 404: hipLaunchKernelGGL((k_eig_lin_batch<4, 1, LOOP>), g, b, 0, st, T); return true;
+        case 804: hipLaunchKernelGGL((k_eig_lin_batch<4, 2, LOOP>), g, b, 0, st, T); return true;
+        case 808: hipLaunchKernelGGL((k_eig_lin_batch<8, 1, LOOP>), g, b, 0, st, T); return true;
+        case 1608: hipLaunchKernelGGL((k_eig_lin_batch<8, 2, LOOP>), g, b, 0, st, T); return true;
+        case 1616: hipLaunchKernelGGL((k_eig_lin_batch<16, 1, LOOP>), g, b, 0, st, T); return true;
       }
       return false;
     case BK_EIG_NOM:
       switch (key) {
-        case 0: hipLaunchKernelGGL(k_eig_batch_fused<true>, g, b, 0, st, T); return true;
-        case 404: hipLaunchKernelGGL((k_eig_nom_batch<4, 1>), g, b, 0, st, T); return true;
-        case 804: hipLaunchKernelGGL((k_eig_nom_batch<4, 2>), g, b, 0, st, T); return true;
-        case 808: hipLaunchKernelGGL((k_eig_nom_batch<8, 1>), g, b, 0, st, T); return true;
-        case 1608: hipLaunchKernelGGL((k_eig_nom_batch<8, 2>), g, b, 0, st, T); return true;
-        case 1616: hipLaunchKernelGGL((k_eig_nom_batch<16, 1>), g, b, 0, st, T); return true;
+        case 0: hipLaunchKernelGGL((k_eig_batch_fused<true, LOOP>), g, b, 0, st, T); return true;
+        case 404: hipLaunchKernelGGL((k_eig_nom_batch<4, 1, LOOP>), g, b, 0, st, T); return true;
+        case 804: hipLaunchKernelGGL((k_eig_nom_batch<4, 2, LOOP>), g, b, 0, st, T); return true;
+        case 808: hipLaunchKernelGGL((k_eig_nom_batch<8, 1, LOOP>), g, b, 0, st, T); return true;
+        case 1608: hipLaunchKernelGGL((k_eig_nom_batch<8, 2, LOOP>), g, b, 0, st, T); return true;
+        case 1616: hipLaunchKernelGGL((k_eig_nom_batch<16, 1, LOOP>), g, b, 0, st, T); return true;
       }
       return false;
-    case BK_COMPACT: hipLaunchKernelGGL(k_compact_batch, g, b, 0, st, T); return true;
+    case BK_COMPACT: hipLaunchKernelGGL(k_compact_batch<LOOP>, g, b, 0, st, T); return true;
     case BK_FIT_FP64:
       switch (key) {
-        case 1: hipLaunchKernelGGL(k_fit_fp64_batch<1>, g, b, 0, st, T); return true;
-        case 2: hipLaunchKernelGGL(k_fit_fp64_batch<2>, g, b, 0, st, T); return true;
-        case 3: hipLaunchKernelGGL(k_fit_fp64_batch<3>, g, b, 0, st, T); return true;
-        case 4: hipLaunchKernelGGL(k_fit_fp64_batch<4>, g, b, 0, st, T); return true;
+        case 1: hipLaunchKernelGGL((k_fit_fp64_batch<1, LOOP>), g, b, 0, st, T); return true;
+        case 2: hipLaunchKernelGGL((k_fit_fp64_batch<2, LOOP>), g, b, 0, st, T); return true;
+        case 3: hipLaunchKernelGGL((k_fit_fp64_batch<3, LOOP>), g, b, 0, st, T); return true;
+        case 4: hipLaunchKernelGGL((k_fit_fp64_batch<4, LOOP>), g, b, 0, st, T); return true;
       }
       return false;
     case BK_SWEEP_FP64: CE_NS_CASES(k_sweep_fp64_batch)
-    case BK_SCAN_FP64: hipLaunchKernelGGL(k_scan_fp64_batch, g, b, 0, st, T); return true;
+    case BK_SCAN_FP64: hipLaunchKernelGGL(k_scan_fp64_batch<LOOP>, g, b, 0, st, T); return true;
   }
   return false;
+#undef CE_NS_CASES
+}
+
+bool launch_batch_fp64(int kind, int key, const BatchTab& T, int grid, hipStream_t st) {
+  return grid >= T.total ? launch_batch_fp64_t<false>(kind, key, T, dim3(grid), st)
+                         : launch_batch_fp64_t<true>(kind, key, T, dim3(grid), st);
 }
 
 void launch_info_init(const float* in, double* out, size_t count, hipStream_t st) {
